@@ -1,0 +1,128 @@
+/*
+ * sdim_hip.h - C ABI of the MI355X (gfx950) prime-dimension stabilizer tableau engine.
+ *
+ * The reference (events555/sdim) is pure Python; its hot path is
+ *   Program.simulate -> Program._simulate_tableau -> apply_gate -> tableau_gates.apply_*
+ *   -> ExtendedTableau.* -> tableau_optimized.*_optimized / ExtendedTableau.measure
+ * (sdim/program.py:206-385, sdim/tableau/tableau_gates.py, sdim/tableau/tableau_prime.py,
+ *  sdim/tableau/tableau_optimized.py).  Each entry point below says which part of that
+ * path it replaces.  Plain C types only; the Python package binds it with ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SDIM_E* code on error;
+ *     sdim_last_error() then describes the error (thread-local string).
+ *   - Inputs are caller-owned and copied during the call; outputs are written into
+ *     caller buffers; device memory is owned by the context.
+ *   - One context may be used from one host thread at a time (calls lock internally).
+ */
+#ifndef SDIM_HIP_H
+#define SDIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDIM_OK 0
+#define SDIM_EINVAL -1     /* bad argument: gate id, qudit index, dimension, ... */
+#define SDIM_EHIP -2       /* a HIP runtime call failed */
+#define SDIM_ENOMEM -3     /* device memory exhausted / over budget */
+#define SDIM_ENOTFOUND -4  /* unknown ticket or job */
+#define SDIM_EBUSY -5      /* ticket not finished (sdim_poll) */
+
+#define SDIM_MODE_BATCHED 0  /* persistent kernel walking device-resident gate lists */
+#define SDIM_MODE_PLAIN 1    /* one kernel launch per primitive op */
+
+typedef struct sdim_ctx sdim_ctx;
+
+/*
+ * One circuit to simulate `shots` times.  Replaces the inputs of
+ * Program.simulate(shots, force_tableau=True) (sdim/program.py:206-267):
+ *   ir          [n_ops][3] int64 (gate_id, qudit_index, target_index | -1), the IR triple
+ *               of Program._build_ir (sdim/program.py:512-516); gate ids follow
+ *               sdim/gatedata.py:42-102 (I0 X1 X_INV2 Z3 Z_INV4 H5 H_INV6 P7 P_INV8 CNOT9
+ *               CNOT_INV10 CZ11 CZ_INV12 SWAP13 M14 M_X15 RESET16 N1 17).
+ *   noise_channel / noise_prob  per op, read for gate 17 only ('d'=0, 'f'=1, 'p'=2; the
+ *               probability that the N1 op applies a non-identity Pauli), may be NULL
+ *               when the circuit has no N1 op.
+ *   seed, stream  Philox4x64-10 key of every random outcome and noise draw; draw
+ *               (op_index, shot) uses counter (op_index, shot, 0, 0).
+ *   init        optional initial tableau in the reference layout (NULL = |0...0>):
+ *               init[0..5] = x, z, ph, dx, dz, dph int64 arrays as in
+ *               sdim/tableau/tableau_prime.py:24-26 (x[qudit*n + generator]).
+ */
+typedef struct {
+  int32_t n;
+  int32_t d;
+  int64_t n_ops;
+  const int64_t* ir;
+  const int32_t* noise_channel;
+  const double* noise_prob;
+  int64_t shots;
+  uint64_t seed;
+  uint64_t stream;
+  const int64_t* init[6];
+} sdim_job;
+
+/* Context on HIP device `device`.  (No reference counterpart: the reference has no device.) */
+int sdim_ctx_create(int32_t device, sdim_ctx** out);
+int sdim_ctx_destroy(sdim_ctx* ctx);
+
+/* Description of the last error on this thread (replaces the reference's ValueError
+ * messages, e.g. "Invalid gate value" sdim/program.py:381-382). */
+const char* sdim_last_error(void);
+
+/*
+ * Tuning / engine options (no reference counterpart).  Keys:
+ *   "threads"         workgroup size of the batch kernel: 256, 512 or 1024
+ *   "lds_bytes"       LDS budget per workgroup (<= 163824)
+ *   "shots_per_item"  shots that share one X/Z block per work item (0 = automatic, 1 = a
+ *                     separate tableau per shot)
+ *   "grid"            persistent grid size (0 = automatic)
+ *   "allow_lds"       0 forces every tableau to stay in HBM
+ */
+int sdim_set_option(sdim_ctx* ctx, const char* key, int64_t value);
+int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
+
+/*
+ * Simulate `n_jobs` circuits.  Replaces Program._simulate_tableau's shot loop
+ * (sdim/program.py:269-365) for many programs at once.  Asynchronous: returns a
+ * ticket; results are available after sdim_wait.
+ */
+int sdim_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket);
+int sdim_wait(sdim_ctx* ctx, uint64_t ticket);
+int sdim_poll(sdim_ctx* ctx, uint64_t ticket, int32_t* done);
+int sdim_release(sdim_ctx* ctx, uint64_t ticket);
+
+/* Number of measurement events of job `job` (M, M_X and RESET ops) - per shot. */
+int sdim_num_measurements(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t* count);
+
+/*
+ * Measurement records of job `job`: shots * M records, event-major then shot, each a
+ * packed 33-byte record byte-compatible with MEASUREMENT_DTYPE (sdim/program.py:34-40):
+ * (int64 qudit_index, int64 meas_round, int64 shot, bool deterministic, int64 value).
+ * Replaces the MeasurementResult bookkeeping of sdim/program.py:321-332.
+ */
+int sdim_fetch_records(sdim_ctx* ctx, uint64_t ticket, int32_t job, void* out, int64_t capacity);
+
+/*
+ * Final tableau of (job, shot) in the reference layout: x, z, dx, dz are n*n int64
+ * (row = qudit, column = generator), ph and dph n int64, all reduced as after
+ * Tableau.modulo (sdim/tableau/dataclasses.py:41-47, tableau_prime.py:88-95).
+ */
+int sdim_fetch_tableau(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t shot,
+                       int64_t* x, int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph);
+
+/*
+ * Timing and traffic of a finished ticket: device time of the simulation kernels
+ * (HIP events on the launch stream) and the tableau bytes the kernels read and
+ * wrote, split by where the tableau lived (HBM-resident vs LDS-resident items).
+ */
+int sdim_ticket_stats(sdim_ctx* ctx, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm,
+                      uint64_t* bytes_lds, int64_t* n_items, int64_t* n_lds_items);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDIM_HIP_H */

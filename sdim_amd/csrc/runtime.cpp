@@ -1,0 +1,783 @@
+// Host runtime of the engine: C ABI (include/sdim_hip.h), circuit lowering,
+// item planning, device memory and result conversion.
+//
+// Lowering restates the reference's gate decompositions for prime d
+// (sdim/tableau/tableau_gates.py): X/X_INV/Z/Z_INV become one closed-form
+// Pauli conjugation (their H/P words of :27-137 net to a phase-only update),
+// CZ/CZ_INV = H^-1(t) SUM(+-1)(c,t) H(t) (:229-261), SWAP = the nine-primitive
+// prime word (:319-328), M_X = H^-1 then M (:280-296), RESET = M then
+// X^((-m) mod d) (sdim/program.py:334-339), N1 = X^a Z^b drawn with the
+// channel semantics of Program._build_ir (sdim/program.py:484-510).
+//
+// Gate runs between measurements become segments; each segment is layered
+// ASAP so that the gates of a layer act on disjoint qudits (gates on disjoint
+// qudits commute exactly, phases included, because their phase updates add).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sdim_hip.h"
+#include "engine.h"
+#include "sdim_types.h"
+
+using namespace sdim;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return fail(SDIM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool supported_dim(int d) { return d == 2 || d == 3 || d == 5 || d == 7 || d == 11; }
+
+uint64_t keep_threshold53(double prob) {
+  // identity iff (u >> 11) < keep53, u the Philox word; matches `u < 1 - prob` (program.py:500-502)
+  const double x = (1.0 - prob) * 9007199254740992.0;
+  if (!(x > 0.0)) return 0;
+  if (x >= 9007199254740992.0) return 1ull << 53;
+  return (uint64_t)std::ceil(x);
+}
+
+struct Prim {
+  uint32_t kind, q, t, a, b, src, aux;  // aux: keep index (noise) / measurement slot
+};
+
+struct JobPlan {
+  int32_t n = 0, d = 0;
+  int64_t shots = 0;
+  int64_t n_meas = 0;
+  std::vector<int32_t> meas_q, meas_round;
+  uint32_t S_item = 1;
+  bool resident = false;
+  uint32_t first_item = 0, n_items = 0;
+  uint32_t ops_off = 0, n_ops = 0;
+  uint64_t keep_off = 0;
+  std::vector<Prim> prims;   // kept for the plain path
+};
+
+struct Alloc {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Ticket {
+  uint64_t id = 0;
+  int mode = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Alloc arena, meta;
+  uint64_t rec_bytes = 0;
+  std::vector<JobPlan> jobs;
+  std::vector<ItemDesc> items;
+  std::vector<uint8_t> h_rec;
+  std::vector<uint8_t> h_meta;                      // staging kept alive until the ticket finishes
+  std::vector<std::vector<uint8_t>> h_inits;
+  unsigned long long* d_counters = nullptr;
+  unsigned long long h_counters[2] = {0, 0};
+  bool finished = false;
+  int64_t n_lds_items = 0;
+};
+
+}  // namespace
+
+struct sdim_ctx {
+  int device = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  uint64_t next_ticket = 1;
+  std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
+  std::multimap<size_t, void*> free_blocks;   // caching allocator
+  // options
+  int64_t threads = 1024;
+  int64_t lds_bytes = 163584;
+  int64_t shots_per_item = 0;
+  int64_t grid = 0;
+  int64_t allow_lds = 1;
+};
+
+namespace {
+
+size_t round_block(size_t b) {
+  size_t r = 1 << 20;
+  while (r < b) r <<= 1;
+  return r;
+}
+
+int ctx_alloc(sdim_ctx* c, size_t bytes, Alloc& a) {
+  const size_t rb = round_block(bytes ? bytes : 1);
+  auto it = c->free_blocks.find(rb);
+  if (it != c->free_blocks.end()) {
+    a.p = it->second;
+    a.bytes = rb;
+    c->free_blocks.erase(it);
+    return 0;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, rb);
+  if (e != hipSuccess) {
+    // drop the cache and retry once
+    for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+    c->free_blocks.clear();
+    e = hipMalloc(&p, rb);
+    if (e != hipSuccess) return fail(SDIM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  a.p = p;
+  a.bytes = rb;
+  return 0;
+}
+
+void ctx_free(sdim_ctx* c, Alloc& a) {
+  if (a.p) c->free_blocks.emplace(a.bytes, a.p);
+  a.p = nullptr;
+  a.bytes = 0;
+}
+
+// ---------------------------------------------------------------- lowering
+int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep) {
+  const int64_t n = j.n;
+  const uint32_t d = (uint32_t)j.d;
+  std::vector<int32_t> counts((size_t)n, 0);
+  jp.prims.clear();
+  jp.meas_q.clear();
+  jp.meas_round.clear();
+  auto push = [&](uint32_t kind, uint32_t q, uint32_t t, uint32_t a, uint32_t b, uint32_t src, uint32_t aux) {
+    jp.prims.push_back(Prim{kind, q, t, a, b, src, aux});
+  };
+  for (int64_t k = 0; k < j.n_ops; k++) {
+    const int64_t gid = j.ir[3 * k], q = j.ir[3 * k + 1], t = j.ir[3 * k + 2];
+    if (gid < 0 || gid > 17) return fail(SDIM_EINVAL, "Invalid gate value");
+    if (q < 0 || q >= n) return fail(SDIM_EINVAL, "qudit index out of range at op " + std::to_string(k));
+    const bool two = (gid >= 9 && gid <= 13);
+    if (two && (t < 0 || t >= n)) return fail(SDIM_EINVAL, "target index out of range at op " + std::to_string(k));
+    const uint32_t Q = (uint32_t)q, Tq = (uint32_t)(two ? t : 0), src = (uint32_t)k;
+    switch (gid) {
+      case 0: break;
+      case 1: push(OP_PAULI, Q, 0, 1 % d, 0, src, 0); break;
+      case 2: push(OP_PAULI, Q, 0, (d - 1) % d, 0, src, 0); break;
+      case 3: push(OP_PAULI, Q, 0, 0, 1 % d, src, 0); break;
+      case 4: push(OP_PAULI, Q, 0, 0, (d - 1) % d, src, 0); break;
+      case 5: push(OP_H, Q, 0, 0, 0, src, 0); break;
+      case 6: push(OP_HI, Q, 0, 0, 0, src, 0); break;
+      case 7: push(OP_P, Q, 0, 0, 0, src, 0); break;
+      case 8: push(OP_PI, Q, 0, 0, 0, src, 0); break;
+      case 9: push(OP_SUM, Q, Tq, 0, 0, src, 0); break;
+      case 10: push(OP_SUMI, Q, Tq, 0, 0, src, 0); break;
+      case 11: push(OP_HI, Tq, 0, 0, 0, src, 0); push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); break;
+      case 12: push(OP_HI, Tq, 0, 0, 0, src, 0); push(OP_SUMI, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); break;
+      case 13:
+        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
+        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
+        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
+        break;
+      case 14: case 15: case 16: {
+        if (gid == 15) push(OP_HI, Q, 0, 0, 0, src, 0);
+        const uint32_t slot = (uint32_t)jp.meas_q.size();
+        push(gid == 16 ? OP_RESET : OP_MEAS, Q, 0, 0, 0, src, slot);
+        jp.meas_q.push_back((int32_t)Q);
+        jp.meas_round.push_back(counts[Q]++);
+        break;
+      }
+      case 17: {
+        if (!j.noise_channel || !j.noise_prob) return fail(SDIM_EINVAL, "N1 op without noise parameters");
+        const int32_t ch = j.noise_channel[k];
+        if (ch < 0 || ch > 2) return fail(SDIM_EINVAL, "Must specify a valid noise channel.");
+        const double p = j.noise_prob[k];
+        const uint32_t kidx = (uint32_t)keep.size();
+        keep.push_back(keep_threshold53(p));
+        push(OP_NOISE, Q, 0, (uint32_t)ch, 0, src, kidx);
+        break;
+      }
+    }
+  }
+  jp.n_meas = (int64_t)jp.meas_q.size();
+  return 0;
+}
+
+DevOp mkop(uint32_t kind, uint32_t a, uint32_t b, uint32_t q, uint32_t t, uint32_t src) {
+  DevOp o;
+  o.w0 = kind | (a << 8) | (b << 16);
+  o.q = q;
+  o.t = t;
+  o.src = src;
+  return o;
+}
+
+// segments + ASAP layers for the batch kernel
+void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max, uint64_t keep_base,
+                    std::vector<DevOp>& out) {
+  (void)keep_base;
+  std::vector<int32_t> last(n, -1);
+  std::vector<uint32_t> touched;
+  size_t i = 0;
+  while (i < prims.size()) {
+    const Prim& p0 = prims[i];
+    if (p0.kind == OP_MEAS || p0.kind == OP_RESET) {
+      out.push_back(mkop(p0.kind, 0, 0, p0.q, p0.aux, p0.src));
+      i++;
+      continue;
+    }
+    size_t j = i;
+    uint32_t nn = 0;
+    while (j < prims.size() && prims[j].kind != OP_MEAS && prims[j].kind != OP_RESET) {
+      if (prims[j].kind == OP_NOISE) {
+        if (nn == nn_max) break;
+        nn++;
+      }
+      j++;
+    }
+    // layer assignment
+    std::vector<std::vector<uint32_t>> layers;
+    for (size_t k = i; k < j; k++) {
+      const Prim& p = prims[k];
+      const bool two = (p.kind == OP_SUM || p.kind == OP_SUMI);
+      int32_t l = last[p.q];
+      if (two) l = std::max(l, last[p.t]);
+      l += 1;
+      if ((size_t)l >= layers.size()) layers.resize((size_t)l + 1);
+      layers[(size_t)l].push_back((uint32_t)k);
+      if (last[p.q] < 0) touched.push_back(p.q);
+      last[p.q] = l;
+      if (two) {
+        if (last[p.t] < 0) touched.push_back(p.t);
+        last[p.t] = l;
+      }
+    }
+    for (uint32_t q : touched) last[q] = -1;
+    touched.clear();
+    out.push_back(mkop(OP_SEG, 0, 0, (uint32_t)layers.size(), nn, 0));
+    std::vector<uint32_t> slot_of(j - i, 0);
+    uint32_t s = 0;
+    for (size_t k = i; k < j; k++) {
+      if (prims[k].kind == OP_NOISE) {
+        slot_of[k - i] = s++;
+        out.push_back(mkop(OP_NOISE, prims[k].a, 0, prims[k].q, prims[k].aux, prims[k].src));
+      }
+    }
+    for (auto& L : layers) {
+      out.push_back(mkop(OP_LAYER, 0, 0, (uint32_t)L.size(), 0, 0));
+      for (uint32_t k : L) {
+        const Prim& p = prims[k];
+        if (p.kind == OP_NOISE) {
+          DevOp o;
+          o.w0 = OP_NOISE | ((slot_of[k - i] & 0xFFFFu) << 16);
+          o.q = p.q;
+          o.t = p.aux;
+          o.src = p.src;
+          out.push_back(o);
+        } else {
+          out.push_back(mkop(p.kind, p.a, p.b, p.q, p.t, p.src));
+        }
+      }
+    }
+    i = j;
+  }
+  out.push_back(mkop(OP_END, 0, 0, 0, 0, 0));
+}
+
+uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+
+// ---------------------------------------------------------------- tableau upload / download
+void pack_tableau(const JobPlan& jp, uint32_t S, const sdim_job& j, std::vector<uint8_t>& buf) {
+  const uint32_t n = (uint32_t)jp.n, d = (uint32_t)jp.d, rs = row_stride(n, d);
+  buf.assign(tableau_bytes(n, d, S), 0);
+  uint8_t* X = buf.data();
+  uint8_t* Z = X + (size_t)n * rs;
+  uint8_t* Ph = Z + (size_t)n * rs;
+  const int64_t *x = j.init[0], *z = j.init[1], *ph = j.init[2], *dx = j.init[3], *dz = j.init[4], *dph = j.init[5];
+  const int64_t ord = (d == 2) ? 4 : d;
+  auto md = [&](int64_t v, int64_t m) { int64_t r = v % m; return r < 0 ? r + m : r; };
+  for (uint32_t r = 0; r < n; r++) {
+    for (uint32_t g = 0; g < 2 * n; g++) {
+      const int64_t xv = md(g < n ? x[(size_t)r * n + g] : dx[(size_t)r * n + g - n], d);
+      const int64_t zv = md(g < n ? z[(size_t)r * n + g] : dz[(size_t)r * n + g - n], d);
+      if (d == 2) {
+        if (xv) ((uint64_t*)(X + (size_t)r * rs))[g >> 6] |= 1ull << (g & 63);
+        if (zv) ((uint64_t*)(Z + (size_t)r * rs))[g >> 6] |= 1ull << (g & 63);
+      } else {
+        X[(size_t)r * rs + g] = (uint8_t)xv;
+        Z[(size_t)r * rs + g] = (uint8_t)zv;
+      }
+    }
+  }
+  for (uint32_t s = 0; s < S; s++) {
+    for (uint32_t g = 0; g < 2 * n; g++) {
+      const int64_t pv = md(g < n ? ph[g] : dph[g - n], ord);
+      if (d == 2) {
+        if (pv & 1) ((uint64_t*)(Ph + (size_t)(2 * s) * rs))[g >> 6] |= 1ull << (g & 63);
+        if (pv & 2) ((uint64_t*)(Ph + (size_t)(2 * s + 1) * rs))[g >> 6] |= 1ull << (g & 63);
+      } else {
+        Ph[(size_t)s * rs + g] = (uint8_t)pv;
+      }
+    }
+  }
+}
+
+void unpack_tableau(const uint8_t* tab, uint32_t n, uint32_t d, uint32_t s, int64_t* x, int64_t* z, int64_t* ph,
+                    int64_t* dx, int64_t* dz, int64_t* dph) {
+  const uint32_t rs = row_stride(n, d);
+  const uint8_t* X = tab;
+  const uint8_t* Z = X + (size_t)n * rs;
+  const uint8_t* Ph = Z + (size_t)n * rs;
+  for (uint32_t r = 0; r < n; r++) {
+    for (uint32_t g = 0; g < 2 * n; g++) {
+      int64_t xv, zv;
+      if (d == 2) {
+        xv = (((const uint64_t*)(X + (size_t)r * rs))[g >> 6] >> (g & 63)) & 1;
+        zv = (((const uint64_t*)(Z + (size_t)r * rs))[g >> 6] >> (g & 63)) & 1;
+      } else {
+        xv = X[(size_t)r * rs + g];
+        zv = Z[(size_t)r * rs + g];
+      }
+      if (g < n) { x[(size_t)r * n + g] = xv; z[(size_t)r * n + g] = zv; }
+      else { dx[(size_t)r * n + g - n] = xv; dz[(size_t)r * n + g - n] = zv; }
+    }
+  }
+  for (uint32_t g = 0; g < 2 * n; g++) {
+    int64_t pv;
+    if (d == 2) {
+      const int64_t b0 = (((const uint64_t*)(Ph + (size_t)(2 * s) * rs))[g >> 6] >> (g & 63)) & 1;
+      const int64_t b1 = (((const uint64_t*)(Ph + (size_t)(2 * s + 1) * rs))[g >> 6] >> (g & 63)) & 1;
+      pv = b0 + 2 * b1;
+    } else {
+      pv = Ph[(size_t)s * rs + g];
+    }
+    if (g < n) ph[g] = pv; else dph[g - n] = pv;
+  }
+}
+
+Ticket* find_ticket(sdim_ctx* c, uint64_t id) {
+  auto it = c->tickets.find(id);
+  return it == c->tickets.end() ? nullptr : it->second.get();
+}
+
+int finish(sdim_ctx* c, Ticket* t) {
+  (void)c;
+  if (t->finished) return 0;
+  HIP_OK(hipStreamSynchronize(t->stream));
+  t->finished = true;
+  return 0;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char* sdim_last_error(void) { return g_err.c_str(); }
+
+int sdim_ctx_create(int32_t device, sdim_ctx** out) {
+  if (!out) return fail(SDIM_EINVAL, "null output pointer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+    return fail(SDIM_EHIP, "no HIP device " + std::to_string(device));
+  HIP_OK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, device));
+  std::unique_ptr<sdim_ctx> c(new sdim_ctx());
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return 0;
+}
+
+int sdim_ctx_destroy(sdim_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tickets) {
+    Ticket* t = kv.second.get();
+    (void)hipStreamSynchronize(t->stream);
+    if (t->arena.p) (void)hipFree(t->arena.p);
+    if (t->meta.p) (void)hipFree(t->meta.p);
+    if (t->ev0) (void)hipEventDestroy(t->ev0);
+    if (t->ev1) (void)hipEventDestroy(t->ev1);
+  }
+  for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
+  if (!c || !key) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const std::string k(key);
+  if (k == "threads") {
+    if (v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "threads must be 256, 512 or 1024");
+    c->threads = v;
+  } else if (k == "lds_bytes") {
+    if (v < 16384 || v > 163584) return fail(SDIM_EINVAL, "lds_bytes out of range");
+    c->lds_bytes = v & ~15ll;
+  } else if (k == "shots_per_item") {
+    if (v < 0) return fail(SDIM_EINVAL, "shots_per_item must be >= 0");
+    c->shots_per_item = v;
+  } else if (k == "grid") {
+    if (v < 0) return fail(SDIM_EINVAL, "grid must be >= 0");
+    c->grid = v;
+  } else if (k == "allow_lds") {
+    c->allow_lds = v ? 1 : 0;
+  } else {
+    return fail(SDIM_EINVAL, "unknown option " + k);
+  }
+  return 0;
+}
+
+int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
+  if (!c || !key || !v) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const std::string k(key);
+  if (k == "threads") *v = c->threads;
+  else if (k == "lds_bytes") *v = c->lds_bytes;
+  else if (k == "shots_per_item") *v = c->shots_per_item;
+  else if (k == "grid") *v = c->grid ? c->grid : c->n_cu;
+  else if (k == "allow_lds") *v = c->allow_lds;
+  else if (k == "num_cu") *v = c->n_cu;
+  else return fail(SDIM_EINVAL, "unknown option " + k);
+  return 0;
+}
+
+int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) {
+  if (!c || (!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
+  if (mode != SDIM_MODE_BATCHED && mode != SDIM_MODE_PLAIN) return fail(SDIM_EINVAL, "unknown mode");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_OK(hipSetDevice(c->device));
+  std::unique_ptr<Ticket> T(new Ticket());
+  T->mode = mode;
+  T->stream = c->stream;
+  T->jobs.resize((size_t)n_jobs);
+  const uint32_t NT = (uint32_t)c->threads;
+  const uint32_t budget = (uint32_t)c->lds_bytes;
+
+  std::vector<uint64_t> keep;
+  std::vector<DevOp> ops;
+  std::vector<ItemDesc>& items = T->items;
+  std::vector<uint64_t> cost;
+  std::vector<std::pair<uint32_t, std::vector<uint8_t>>> inits;   // item -> initial tableau bytes
+
+  uint64_t rec_total = 0;
+  for (int32_t ji = 0; ji < n_jobs; ji++) {
+    const sdim_job& j = jobs[ji];
+    JobPlan& jp = T->jobs[(size_t)ji];
+    if (j.n < 1) return fail(SDIM_EINVAL, "Number of qudits must be greater than 0");
+    if (j.d < 2) return fail(SDIM_EINVAL, "Dimension must be greater than 1");
+    if (!supported_dim(j.d)) return fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})");
+    if (j.shots < 1) return fail(SDIM_EINVAL, "shots must be >= 1");
+    if (j.n_ops < 0 || (j.n_ops && !j.ir)) return fail(SDIM_EINVAL, "bad op list");
+    jp.n = j.n;
+    jp.d = j.d;
+    jp.shots = j.shots;
+    jp.keep_off = keep.size();
+    const size_t keep0 = keep.size();
+    int rc = expand(j, jp, keep);
+    if (rc) return rc;
+    for (size_t k = keep0; k < keep.size(); k++) (void)k;
+    // keep indices in prims are global; make them circuit-relative
+    for (auto& p : jp.prims) if (p.kind == OP_NOISE) p.aux -= (uint32_t)keep0;
+
+    const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = row_stride(n, d);
+    const bool has_noise = keep.size() > keep0;
+    // shots per item
+    uint32_t S = 1;
+    bool resident = false;
+    const uint32_t want = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
+    if (mode == SDIM_MODE_BATCHED && c->allow_lds) {
+      // largest S <= want with tableau + scratch (+ one noise op) inside the LDS budget
+      uint32_t lo = 0, hi = want;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        const uint64_t need = ((tableau_bytes(n, d, mid) + 15) & ~15ull) + scratch_bytes(n, rs, mid, NT) +
+                              (has_noise ? noise_bytes(1, rs, mid) : 0);
+        if (need <= budget) lo = mid; else hi = mid - 1;
+      }
+      if (lo >= 1 && (c->shots_per_item ? lo >= want : true)) { S = lo; resident = true; }
+    }
+    if (!resident) {
+      S = want;
+      if (mode == SDIM_MODE_BATCHED) {
+        // scratch (+ one noise op) of an HBM-resident item must fit in LDS
+        while (S > 1 && scratch_bytes(n, rs, S, NT) + (has_noise ? noise_bytes(1, rs, S) : 0) > budget) S /= 2;
+        if (scratch_bytes(n, rs, S, NT) + (has_noise ? noise_bytes(1, rs, S) : 0) > budget)
+          return fail(SDIM_ENOMEM, "circuit too large for the LDS scratch budget");
+      } else {
+        if (scratch_bytes(n, rs, S, 256) > 163584) return fail(SDIM_ENOMEM, "circuit too large for the plain path");
+      }
+    }
+    jp.S_item = S;
+    jp.resident = resident;
+    // noise ops per segment that fit next to the tableau / scratch
+    uint32_t nn_max = 0;
+    if (mode == SDIM_MODE_BATCHED) {
+      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, S, NT);
+      const uint64_t room = budget > base ? budget - base : 0;
+      while (nn_max < 65535 && noise_bytes(nn_max + 1, rs, S) <= room) nn_max++;
+      if (has_noise && nn_max == 0) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
+      jp.ops_off = (uint32_t)ops.size();
+      build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, ops);
+      jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
+    }
+    // items
+    jp.first_item = (uint32_t)items.size();
+    const uint64_t tb = tableau_bytes(n, d, S);
+    std::vector<uint8_t> init_buf;
+    const bool given = j.init[0] && j.init[1] && j.init[2] && j.init[3] && j.init[4] && j.init[5];
+    if (given) pack_tableau(jp, S, j, init_buf);
+    for (int64_t s0 = 0; s0 < j.shots; s0 += S) {
+      ItemDesc it;
+      memset(&it, 0, sizeof(it));
+      it.seed = j.seed;
+      it.stream = j.stream;
+      it.keep_off = jp.keep_off;
+      it.ops_off = jp.ops_off;
+      it.n_ops = jp.n_ops;
+      it.n = n;
+      it.d = d;
+      it.S = (uint32_t)std::min<int64_t>(S, j.shots - s0);
+      it.shot0 = (uint32_t)s0;
+      it.rs = rs;
+      it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0);
+      it.n_meas = (uint32_t)jp.n_meas;
+      it.rec_off = rec_total;
+      rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
+      if (given) inits.emplace_back((uint32_t)items.size(), init_buf);
+      items.push_back(it);
+      // LPT cost: measurement sweeps dominate (rows * row bytes * 3 per measurement) + phase passes
+      const uint64_t sweep = (uint64_t)jp.n_meas * 3ull * n * rs + (uint64_t)it.S * rs * (jp.n_meas + 4) + jp.prims.size() * 8ull * rs;
+      cost.push_back(resident ? sweep / 6 : sweep);
+      (void)tb;
+    }
+    jp.n_items = (uint32_t)items.size() - jp.first_item;
+  }
+  // arena: records first, then tableaux
+  T->rec_bytes = rec_total;
+  uint64_t off = align256(rec_total);
+  for (auto& it : items) {
+    it.tab_off = off;
+    off = align256(off + tableau_bytes(it.n, it.d, it.S));
+  }
+  const uint64_t arena_bytes = std::max<uint64_t>(off, 256);
+  // metadata: items, order, ops, keep, queue, counters
+  std::vector<uint32_t> order(items.size());
+  for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  const uint64_t o_items = 0;
+  const uint64_t o_order = align256(o_items + items.size() * sizeof(ItemDesc));
+  const uint64_t o_ops = align256(o_order + order.size() * 4);
+  const uint64_t o_keep = align256(o_ops + ops.size() * sizeof(DevOp));
+  const uint64_t o_queue = align256(o_keep + keep.size() * 8);
+  const uint64_t o_cnt = align256(o_queue + 16);
+  const uint64_t meta_bytes = align256(o_cnt + 16);
+  int rc = ctx_alloc(c, arena_bytes, T->arena);
+  if (rc) return rc;
+  rc = ctx_alloc(c, meta_bytes, T->meta);
+  if (rc) { ctx_free(c, T->arena); return rc; }
+  uint8_t* meta = (uint8_t*)T->meta.p;
+  uint8_t* arena = (uint8_t*)T->arena.p;
+  std::vector<uint8_t>& h_meta = T->h_meta;
+  h_meta.assign(meta_bytes, 0);
+  if (!items.empty()) memcpy(h_meta.data() + o_items, items.data(), items.size() * sizeof(ItemDesc));
+  if (!order.empty()) memcpy(h_meta.data() + o_order, order.data(), order.size() * 4);
+  if (!ops.empty()) memcpy(h_meta.data() + o_ops, ops.data(), ops.size() * sizeof(DevOp));
+  if (!keep.empty()) memcpy(h_meta.data() + o_keep, keep.data(), keep.size() * 8);
+  HIP_OK(hipMemcpyAsync(meta, h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
+  for (auto& kv : inits) {
+    const ItemDesc& it = items[kv.first];
+    T->h_inits.push_back(std::move(kv.second));
+    HIP_OK(hipMemcpyAsync(arena + it.tab_off, T->h_inits.back().data(), T->h_inits.back().size(),
+                          hipMemcpyHostToDevice, T->stream));
+  }
+  T->d_counters = (unsigned long long*)(meta + o_cnt);
+  HIP_OK(hipEventCreate(&T->ev0));
+  HIP_OK(hipEventCreate(&T->ev1));
+
+  KParams P;
+  P.items = (const ItemDesc*)(meta + o_items);
+  P.order = (const uint32_t*)(meta + o_order);
+  P.n_items = (uint32_t)items.size();
+  P.queue = (uint32_t*)(meta + o_queue);
+  P.ops = (const DevOp*)(meta + o_ops);
+  P.keep53 = (const uint64_t*)(meta + o_keep);
+  P.arena = arena;
+  P.counters = T->d_counters;
+  P.lds_bytes = budget;
+  for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
+
+  HIP_OK(hipEventRecord(T->ev0, T->stream));
+  if (mode == SDIM_MODE_BATCHED) {
+    if (!items.empty()) {
+      const int grid = (int)(c->grid ? c->grid : c->n_cu);
+      rc = launch_batch(P, (int)NT, grid, T->stream);
+      if (rc) return fail(SDIM_EHIP, "batch kernel launch failed");
+    }
+  } else {
+    for (auto& jp : T->jobs) {
+      KParams Q = P;
+      Q.items = P.items + jp.first_item;
+      Q.n_items = jp.n_items;
+      uint32_t Smax = 0;
+      for (uint32_t i = 0; i < jp.n_items; i++) Smax = std::max(Smax, items[jp.first_item + i].S);
+      const bool given = items[jp.first_item].flags & ITEM_INIT_GIVEN;
+      if (!given) {
+        rc = launch_plain_op(Q, mkop(OP_END, 0, 0, 0, 0, 0), (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream);
+        if (rc) return fail(SDIM_EHIP, "plain init launch failed");
+      }
+      for (const Prim& p : jp.prims) {
+        DevOp o;
+        if (p.kind == OP_NOISE) o = mkop(OP_NOISE, p.a, 0, p.q, p.aux, p.src);
+        else if (p.kind == OP_MEAS || p.kind == OP_RESET) o = mkop(p.kind, 0, 0, p.q, p.aux, p.src);
+        else o = mkop(p.kind, p.a, p.b, p.q, p.t, p.src);
+        rc = launch_plain_op(Q, o, (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream);
+        if (rc) return fail(SDIM_EHIP, "plain op launch failed");
+      }
+    }
+  }
+  HIP_OK(hipEventRecord(T->ev1, T->stream));
+  // records back to the host (async; valid after wait)
+  T->h_rec.resize(T->rec_bytes);
+  if (T->rec_bytes) HIP_OK(hipMemcpyAsync(T->h_rec.data(), arena, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
+  HIP_OK(hipMemcpyAsync(T->h_counters, T->d_counters, 16, hipMemcpyDeviceToHost, T->stream));
+
+  T->id = c->next_ticket++;
+  *ticket = T->id;
+  c->tickets[T->id] = std::move(T);
+  return 0;
+}
+
+int sdim_wait(sdim_ctx* c, uint64_t ticket) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  return finish(c, t);
+}
+
+int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) {
+  if (!c || !done) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  if (t->finished) { *done = 1; return 0; }
+  const hipError_t e = hipEventQuery(t->ev1);
+  if (e == hipErrorNotReady) { *done = 0; return 0; }
+  if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  *done = 1;
+  return finish(c, t);
+}
+
+int sdim_release(sdim_ctx* c, uint64_t ticket) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->tickets.find(ticket);
+  if (it == c->tickets.end()) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  Ticket* t = it->second.get();
+  int rc = finish(c, t);
+  ctx_free(c, t->arena);
+  ctx_free(c, t->meta);
+  if (t->ev0) (void)hipEventDestroy(t->ev0);
+  if (t->ev1) (void)hipEventDestroy(t->ev1);
+  c->tickets.erase(it);
+  return rc;
+}
+
+int sdim_num_measurements(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t* count) {
+  if (!c || !count) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  if (job < 0 || (size_t)job >= t->jobs.size()) return fail(SDIM_ENOTFOUND, "unknown job");
+  *count = t->jobs[(size_t)job].n_meas;
+  return 0;
+}
+
+int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int64_t capacity) {
+  if (!c || !out) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  if (job < 0 || (size_t)job >= t->jobs.size()) return fail(SDIM_ENOTFOUND, "unknown job");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  const JobPlan& jp = t->jobs[(size_t)job];
+  if (capacity < jp.n_meas * jp.shots) return fail(SDIM_EINVAL, "record buffer too small");
+  uint8_t* o = (uint8_t*)out;
+  for (int64_t ev = 0; ev < jp.n_meas; ev++) {
+    for (uint32_t i = 0; i < jp.n_items; i++) {
+      const ItemDesc& it = t->items[jp.first_item + i];
+      const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
+      for (uint32_t s = 0; s < it.S; s++) {
+        const int64_t shot = (int64_t)it.shot0 + s;
+        uint8_t* rec = o + 33 * (size_t)(ev * jp.shots + shot);
+        const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], val = r[s] & 0x7F;
+        const uint8_t det = (r[s] >> 7) & 1;
+        memcpy(rec, &q, 8);
+        memcpy(rec + 8, &round, 8);
+        memcpy(rec + 16, &shot, 8);
+        rec[24] = det;
+        memcpy(rec + 25, &val, 8);
+      }
+    }
+  }
+  return 0;
+}
+
+int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, int64_t* x, int64_t* z,
+                       int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) {
+  if (!c || !x || !z || !ph || !dx || !dz || !dph) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  if (job < 0 || (size_t)job >= t->jobs.size()) return fail(SDIM_ENOTFOUND, "unknown job");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  const JobPlan& jp = t->jobs[(size_t)job];
+  if (shot < 0 || shot >= jp.shots) return fail(SDIM_EINVAL, "shot out of range");
+  for (uint32_t i = 0; i < jp.n_items; i++) {
+    const ItemDesc& it = t->items[jp.first_item + i];
+    if (shot >= it.shot0 && shot < (int64_t)it.shot0 + it.S) {
+      std::vector<uint8_t> buf(tableau_bytes(it.n, it.d, it.S));
+      HIP_OK(hipMemcpy(buf.data(), (uint8_t*)t->arena.p + it.tab_off, buf.size(), hipMemcpyDeviceToHost));
+      unpack_tableau(buf.data(), it.n, it.d, (uint32_t)(shot - it.shot0), x, z, ph, dx, dz, dph);
+      return 0;
+    }
+  }
+  return fail(SDIM_ENOTFOUND, "shot not found");
+}
+
+int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm, uint64_t* bytes_lds,
+                      int64_t* n_items, int64_t* n_lds_items) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+  if (kernel_ms) *kernel_ms = ms;
+  if (bytes_hbm) *bytes_hbm = t->h_counters[0];
+  if (bytes_lds) *bytes_lds = t->h_counters[1];
+  if (n_items) *n_items = (int64_t)t->items.size();
+  if (n_lds_items) *n_lds_items = t->n_lds_items;
+  return 0;
+}
+
+}  // extern "C"

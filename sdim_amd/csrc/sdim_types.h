@@ -1,0 +1,83 @@
+// Shared host/device encodings of the engine: lowered device ops, item
+// descriptors and the tableau layout.
+//
+// Tableau layout of one item (one circuit, S shots sharing the X/Z blocks):
+//   generators are COLUMNS (sdim/tableau/dataclasses.py:14-15): stabilizers
+//   0..n-1 then destabilizers n..2n-1, G = 2n columns; rows are qudits.
+//   odd d : X[n][rs], Z[n][rs] uint8 (one generator per byte, value in [0,d)),
+//           Ph[S][rs] uint8 phases mod d;  rs = round_up(G, 16) bytes.
+//   d = 2 : X[n][rs], Z[n][rs] bit-packed (generator g = bit g%64 of u64 word
+//           g/64), Ph[S][2][rs] two bit-planes p0,p1 with phase = p0 + 2 p1
+//           (mod 4);  rs = 8 * round_up(ceil(G/64), 2) bytes.
+//   Bytes of an item: T = 2 n rs + S rs (odd) or 2 n rs + 2 S rs (d = 2).
+// The X/Z blocks are identical for every shot of a circuit (only the phases
+// depend on the random outcomes and noise draws), so an item stores them once
+// and keeps one phase vector per shot.
+#pragma once
+#include <stdint.h>
+
+namespace sdim {
+
+enum OpKind : uint32_t {
+  OP_H = 0, OP_HI = 1, OP_P = 2, OP_PI = 3, OP_SUM = 4, OP_SUMI = 5,
+  OP_PAULI = 6,     // X^a Z^b conjugation, a = byte1, b = byte2 (shot-invariant)
+  OP_NOISE = 7,     // N1: channel = byte1, keep53 index = t, noise slot in segment = byte2|byte3<<8
+  OP_MEAS = 8,      // t = measurement slot
+  OP_RESET = 9,     // t = measurement slot; then X^((-m) mod d)
+  OP_SEG = 10,      // gate segment header: q = #layers, t = #noise ops in the segment
+  OP_LAYER = 11,    // layer header: q = #ops in the layer
+  OP_END = 12,
+};
+
+struct DevOp {
+  uint32_t w0;   // kind | a << 8 | b << 16 | c << 24
+  uint32_t q;
+  uint32_t t;
+  uint32_t src;  // instruction index in the user's circuit (Philox counter word 0)
+};
+
+static inline __host__ __device__ uint32_t op_kind(const DevOp& o) { return o.w0 & 0xFFu; }
+static inline __host__ __device__ uint32_t op_a(const DevOp& o) { return (o.w0 >> 8) & 0xFFu; }
+static inline __host__ __device__ uint32_t op_b(const DevOp& o) { return (o.w0 >> 16) & 0xFFu; }
+static inline __host__ __device__ uint32_t op_c(const DevOp& o) { return (o.w0 >> 24) & 0xFFu; }
+static inline __host__ __device__ uint32_t op_noise_slot(const DevOp& o) { return (o.w0 >> 16) & 0xFFFFu; }
+
+enum ItemFlags : uint32_t {
+  ITEM_LDS = 1u,        // tableau resident in LDS for the whole item
+  ITEM_INIT_GIVEN = 2u, // start from the tableau already stored at tab_off (else identity)
+};
+
+struct ItemDesc {
+  uint64_t tab_off;    // byte offset of the item's tableau in the arena (final state lands here)
+  uint64_t rec_off;    // byte offset of records: [n_meas][S] uint8 = value | det << 7
+  uint64_t seed;       // Philox key word 0
+  uint64_t stream;     // Philox key word 1
+  uint64_t keep_off;   // index of the circuit's first keep53 threshold
+  uint32_t ops_off;    // first DevOp of the circuit in the op pool
+  uint32_t n_ops;
+  uint32_t n;
+  uint32_t d;
+  uint32_t S;          // shots in this item
+  uint32_t shot0;      // global index of the first shot (Philox counter word 1)
+  uint32_t rs;         // row stride in bytes
+  uint32_t flags;
+  uint32_t n_meas;
+  uint32_t pad;
+};
+
+static inline __host__ __device__ uint32_t row_stride(uint32_t n, uint32_t d) {
+  uint32_t G = 2 * n;
+  if (d == 2) {
+    uint32_t w = (G + 63) / 64;
+    w = (w + 1) & ~1u;
+    return 8 * w;
+  }
+  return (G + 15) & ~15u;
+}
+
+static inline __host__ __device__ uint64_t tableau_bytes(uint32_t n, uint32_t d, uint32_t S) {
+  uint64_t rs = row_stride(n, d);
+  return 2ull * n * rs + (d == 2 ? 2ull : 1ull) * S * rs;
+}
+
+}  // namespace sdim
