@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the prime-d stabilizer tableau engine on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload uneven|small|repetition]
+
+A step is one pass of the hot path over the whole workload: every circuit of
+the batch simulated for all its shots (records and final tableaux written),
+with every input already resident in HBM (the captured hipGraph of the batched
+path is replayed).  Prints ONE JSON line.
+
+Besides the headline (batched persistent path) the line carries:
+  plain          the one-launch-per-primitive-op path on a deterministic subset of
+                 the circuits (every k-th circuit), and the batched path on the same subset
+  roofline       HBM roofline of the kernel pass over HBM-resident tableaux
+  roofline_lds   LDS roofline of the kernel pass over LDS-resident tableaux
+  cpu_baseline   the CPU oracle (literal restatement of the reference's
+                 Program.simulate tableau path, same interpreted cost structure),
+                 timed on a bounded sample on this host, 1 core
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "gate-shots/s, batched vs plain on 1 MI355X (bit-exact); p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="uneven", choices=["uneven", "small", "repetition"])
+    ap.add_argument("--plain-every", type=int, default=16, help="plain path runs every k-th circuit")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--shots-per-item", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def make_workload(name):
+    from sdim_amd import workloads as W
+    if name == "uneven":
+        return W.uneven_batch()
+    if name == "small":
+        return W.plain_small()
+    return W.repetition_code()
+
+
+def cpu_baseline(jobs, budget_s):
+    """Literal CPU oracle (reference cost structure) on the first circuits of the workload, 1 shot each."""
+    from oracle import philox
+    from oracle import program as OP
+    done_ops, done_jobs, t_total = 0, 0, 0.0
+    for j in jobs:
+        ir = np.asarray(j.ir).reshape(-1, 3)
+        noise = {}
+        if j.noise_channel is not None:
+            for k in np.nonzero(ir[:, 0] == 17)[0]:
+                noise[int(k)] = (int(j.noise_channel[k]), philox.keep_threshold53(float(j.noise_prob[k])))
+        t0 = time.perf_counter()
+        OP.simulate(j.n, j.d, [tuple(r) for r in ir.tolist()], shots=1, seed=j.seed, stream=j.stream,
+                    noise=noise, keep_tableaux=False)
+        t_total += time.perf_counter() - t0
+        done_ops += int((ir[:, 0] != 0).sum())
+        done_jobs += 1
+        if t_total >= budget_s:
+            break
+    ns = [int(j.n) for j in jobs[:done_jobs]]
+    return {"value": done_ops / t_total, "unit": "gate-shots/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/program.py (literal restatement of sdim Program.simulate tableau path, "
+                      f"interpreted per-gate loops) on the first {done_jobs} circuit(s) of the workload "
+                      f"(n={ns}), 1 shot each, {done_ops} gate-shots in {t_total:.1f} s",
+            "host_cores_available": len(os.sched_getaffinity(0))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
+                                   "(sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    from sdim_amd.engine import Engine
+
+    wl = make_workload(args.workload)
+    jobs = wl.jobs
+    gate_shots = wl.gate_shots
+    opts = {}
+    if args.shots_per_item:
+        opts["shots_per_item"] = args.shots_per_item
+    if args.threads:
+        opts["threads"] = args.threads
+    eng = Engine(**opts)
+
+    t_submit = time.perf_counter()
+    tk = eng.submit(jobs, "batched").wait()
+    t_submit = time.perf_counter() - t_submit
+    st = tk.stats()
+
+    if args.warmup:
+        tk.replay(args.warmup)
+    t0 = time.perf_counter()
+    dev_ms = tk.replay(args.steps)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = gate_shots / (wall / args.steps)
+
+    # per-class kernel timing (HIP events on the launch stream)
+    classes = {}
+    for which in ("hbm", "lds"):
+        n_cls = st["items"] - st["lds_items"] if which == "hbm" else st["lds_items"]
+        if n_cls == 0:
+            continue
+        ms = tk.replay(args.steps, which) / args.steps
+        s2 = tk.stats()
+        classes[which] = {"ms": ms, "bytes": s2["bytes_hbm"] if which == "hbm" else s2["bytes_lds"], "items": n_cls}
+
+    def roof(cls, peak):
+        if cls not in classes:
+            return None
+        c = classes[cls]
+        ach = c["bytes"] / (c["ms"] * 1e-3) / 1e9
+        return {"bound": "hbm" if cls == "hbm" else "lds", "achieved": round(ach, 1), "peak": peak,
+                "unit": "GB/s", "frac": round(ach / peak, 4), "traffic": None,
+                "kernel": "sdim::batch_kernel<%d>" % eng.get_option("threads"),
+                "items": c["items"], "algorithmic_bytes": c["bytes"], "kernel_ms": round(c["ms"], 3)}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "gate-shots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8,u64", "data": "synthetic",
+        "config": {"workload": wl.name, "description": wl.description, "circuits": len(jobs),
+                   "gate_shots": gate_shots, "items": st["items"], "lds_items": st["lds_items"],
+                   "threads": eng.get_option("threads"), "shots_per_item": eng.get_option("shots_per_item")},
+        "device_ms_per_step": dev_ms / args.steps,
+        "first_submit_s": t_submit,
+        "roofline": roof("hbm", HBM_PEAK_GBS) or roof("lds", LDS_PEAK_GBS),
+        "roofline_lds": roof("lds", LDS_PEAK_GBS),
+        "dominant_kernel": max(classes, key=lambda k: classes[k]["ms"]) if classes else None,
+    }
+    tk.release()
+
+    if not args.no_plain:
+        sub = jobs[::args.plain_every] if len(jobs) > 1 else jobs
+        from sdim_amd.workloads import count_gate_shots
+        gs = count_gate_shots(sub)
+        tp = eng.submit(sub, "plain").wait()
+        t0 = time.perf_counter()
+        tp.replay(1)
+        t_plain = time.perf_counter() - t0
+        tp.release()
+        tb = eng.submit(sub, "batched").wait()
+        tb.replay(1)
+        t0 = time.perf_counter()
+        tb.replay(args.steps)
+        t_b = (time.perf_counter() - t0) / args.steps
+        tb.release()
+        out["plain"] = {"value": gs / t_plain, "unit": "gate-shots/s", "circuits": len(sub),
+                        "gate_shots": gs, "seconds": t_plain,
+                        "sample": f"every {args.plain_every}-th circuit of the workload",
+                        "batched_same_subset": gs / t_b}
+        out["speedup_vs_plain"] = value / (gs / t_plain)
+        out["speedup_vs_plain_same_subset"] = (gs / t_b) / (gs / t_plain)
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -122,6 +122,17 @@ int sdim_fetch_tableau(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t shot
 int sdim_ticket_stats(sdim_ctx* ctx, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm,
                       uint64_t* bytes_lds, int64_t* n_items, int64_t* n_lds_items);
 
+/*
+ * Re-run the device work of a finished ticket `iters` times back to back, with every
+ * input already resident in HBM (the batched path replays a captured hipGraph: reset
+ * kernel + persistent kernels), and return the device time of the whole run in ms
+ * (HIP events on the launch stream).  which: 0 = everything, 1 = only HBM-resident
+ * items, 2 = only LDS-resident items (batched mode).  Results are recomputed
+ * identically (every draw is counter-based).  No reference counterpart: used to time
+ * the hot path without host-side lowering and copies.
+ */
+int sdim_replay(sdim_ctx* ctx, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

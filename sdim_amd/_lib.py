@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libsdim_hip.so")
 EXPORTS = (
     "sdim_ctx_create", "sdim_ctx_destroy", "sdim_last_error", "sdim_set_option", "sdim_get_option",
     "sdim_submit", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
-    "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats",
+    "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats", "sdim_replay",
 )
 
 MODE_BATCHED = 0
@@ -67,6 +67,7 @@ def load(path=None):
     L.sdim_fetch_tableau.argtypes = [vp, u64, i32, i64, vp, vp, vp, vp, vp, vp]
     L.sdim_ticket_stats.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                     ctypes.POINTER(u64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.sdim_replay.argtypes = [vp, u64, i32, i32, ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTS:
         f = getattr(L, name)
         if name != "sdim_last_error":

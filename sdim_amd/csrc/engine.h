@@ -37,6 +37,8 @@ __host__ __device__ inline uint32_t noise_bytes(uint32_t nn, uint32_t rs, uint32
   return b;
 }
 
+int prepare_kernels();
+int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st);
 int launch_plain_op(const KParams& P, const DevOp& o, uint32_t n, uint32_t d, uint32_t n_items, uint32_t S,
                     hipStream_t st);

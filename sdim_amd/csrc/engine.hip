@@ -992,21 +992,34 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
 // ======================================================================== host launchers
 namespace sdim {
 
+__global__ void reset_kernel(uint32_t* queue, unsigned long long* counters) {
+  if (threadIdx.x < 4) queue[threadIdx.x] = 0;
+  if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+}
+
+int prepare_kernels() {
+  const int lds = 163584;
+  if (hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+#define SDIM_SET_PLAIN(D) \
+  if (hipFuncSetAttribute((const void*)plain_meas_kernel<D, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  SDIM_SET_PLAIN(2) SDIM_SET_PLAIN(3) SDIM_SET_PLAIN(5) SDIM_SET_PLAIN(7) SDIM_SET_PLAIN(11)
+#undef SDIM_SET_PLAIN
+  return 0;
+}
+
+int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st) {
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, queue, counters);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st) {
   const size_t lds = P.lds_bytes;
   switch (nt) {
-    case 256:
-      if (hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
-      hipLaunchKernelGGL(batch_kernel<256>, dim3(grid), dim3(256), lds, st, P);
-      break;
-    case 512:
-      if (hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
-      hipLaunchKernelGGL(batch_kernel<512>, dim3(grid), dim3(512), lds, st, P);
-      break;
-    case 1024:
-      if (hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
-      hipLaunchKernelGGL(batch_kernel<1024>, dim3(grid), dim3(1024), lds, st, P);
-      break;
+    case 256: hipLaunchKernelGGL(batch_kernel<256>, dim3(grid), dim3(256), lds, st, P); break;
+    case 512: hipLaunchKernelGGL(batch_kernel<512>, dim3(grid), dim3(512), lds, st, P); break;
+    case 1024: hipLaunchKernelGGL(batch_kernel<1024>, dim3(grid), dim3(1024), lds, st, P); break;
     default: return -2;
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -1017,9 +1030,6 @@ static int plain_op_d(const KParams& P, const DevOp& o, uint32_t n, uint32_t n_i
   const uint32_t kind = op_kind(o);
   if (kind == OP_MEAS || kind == OP_RESET) {
     const uint32_t lds = scratch_bytes(n, row_stride(n, D), S, 256);
-    if (lds > 65536) {
-      if (hipFuncSetAttribute((const void*)plain_meas_kernel<D, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
-    }
     hipLaunchKernelGGL((plain_meas_kernel<D, 256>), dim3(n_items), dim3(256), lds, st, P, o);
   } else if (kind == OP_END) {
     hipLaunchKernelGGL(plain_init_kernel<D>, dim3(n_items), dim3(256), 0, st, P);

@@ -94,6 +94,12 @@ struct Ticket {
   unsigned long long h_counters[2] = {0, 0};
   bool finished = false;
   int64_t n_lds_items = 0;
+  // launch plan (re-enqueued by sdim_replay)
+  KParams P{};
+  uint32_t n_hbm_items = 0;
+  uint32_t* d_queue = nullptr;
+  int nt = 1024, grid = 256;
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 };
 
 }  // namespace
@@ -377,6 +383,51 @@ int finish(sdim_ctx* c, Ticket* t) {
   return 0;
 }
 
+// Enqueue the device work of a ticket on its stream.  which: 0 all items,
+// 1 HBM-resident items only, 2 LDS-resident items only (batched mode).
+int enqueue(Ticket* T, int which) {
+  KParams& P = T->P;
+  if (T->mode == SDIM_MODE_BATCHED) {
+    if (launch_reset(T->d_queue, P.counters, T->stream)) return fail(SDIM_EHIP, "reset launch failed");
+    if (T->n_hbm_items && which != 2) {
+      KParams Q = P;
+      Q.n_items = T->n_hbm_items;
+      Q.queue = T->d_queue;
+      if (launch_batch(Q, T->nt, T->grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
+    }
+    if (T->n_lds_items && which != 1) {
+      KParams Q = P;
+      Q.order = P.order + T->n_hbm_items;
+      Q.n_items = (uint32_t)T->n_lds_items;
+      Q.queue = T->d_queue + 1;
+      if (launch_batch(Q, T->nt, T->grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
+    }
+    return 0;
+  }
+  if (which != 0) return fail(SDIM_EINVAL, "the plain path has no item classes");
+  for (auto& jp : T->jobs) {
+    KParams Q = P;
+    Q.items = P.items + jp.first_item;
+    Q.n_items = jp.n_items;
+    uint32_t Smax = 0;
+    for (uint32_t i = 0; i < jp.n_items; i++) Smax = std::max(Smax, T->items[jp.first_item + i].S);
+    const bool given = T->items[jp.first_item].flags & ITEM_INIT_GIVEN;
+    if (!given) {
+      if (launch_plain_op(Q, mkop(OP_END, 0, 0, 0, 0, 0), (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream))
+        return fail(SDIM_EHIP, "plain init launch failed");
+    }
+    for (const Prim& p : jp.prims) {
+      DevOp o;
+      if (p.kind == OP_NOISE) o = mkop(OP_NOISE, p.a, 0, p.q, p.aux, p.src);
+      else if (p.kind == OP_MEAS || p.kind == OP_RESET) o = mkop(p.kind, 0, 0, p.q, p.aux, p.src);
+      else o = mkop(p.kind, p.a, p.b, p.q, p.t, p.src);
+      if (launch_plain_op(Q, o, (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream))
+        return fail(SDIM_EHIP, "plain op launch failed");
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 // ==================================================================== C ABI
@@ -396,6 +447,7 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (prepare_kernels()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
   *out = c.release();
   return 0;
 }
@@ -578,7 +630,11 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   // metadata: items, order, ops, keep, queue, counters
   std::vector<uint32_t> order(items.size());
   for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    const bool ra = items[a].flags & ITEM_LDS, rb = items[b].flags & ITEM_LDS;
+    if (ra != rb) return !ra;          // HBM-resident items first
+    return cost[a] > cost[b];          // longest first inside a class
+  });
   const uint64_t o_items = 0;
   const uint64_t o_order = align256(o_items + items.size() * sizeof(ItemDesc));
   const uint64_t o_ops = align256(o_order + order.size() * 4);
@@ -609,7 +665,7 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   HIP_OK(hipEventCreate(&T->ev0));
   HIP_OK(hipEventCreate(&T->ev1));
 
-  KParams P;
+  KParams& P = T->P;
   P.items = (const ItemDesc*)(meta + o_items);
   P.order = (const uint32_t*)(meta + o_order);
   P.n_items = (uint32_t)items.size();
@@ -619,37 +675,15 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   P.arena = arena;
   P.counters = T->d_counters;
   P.lds_bytes = budget;
+  T->d_queue = P.queue;
   for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
+  T->n_hbm_items = (uint32_t)items.size() - (uint32_t)T->n_lds_items;
+  T->nt = (int)NT;
+  T->grid = (int)(c->grid ? c->grid : c->n_cu);
 
   HIP_OK(hipEventRecord(T->ev0, T->stream));
-  if (mode == SDIM_MODE_BATCHED) {
-    if (!items.empty()) {
-      const int grid = (int)(c->grid ? c->grid : c->n_cu);
-      rc = launch_batch(P, (int)NT, grid, T->stream);
-      if (rc) return fail(SDIM_EHIP, "batch kernel launch failed");
-    }
-  } else {
-    for (auto& jp : T->jobs) {
-      KParams Q = P;
-      Q.items = P.items + jp.first_item;
-      Q.n_items = jp.n_items;
-      uint32_t Smax = 0;
-      for (uint32_t i = 0; i < jp.n_items; i++) Smax = std::max(Smax, items[jp.first_item + i].S);
-      const bool given = items[jp.first_item].flags & ITEM_INIT_GIVEN;
-      if (!given) {
-        rc = launch_plain_op(Q, mkop(OP_END, 0, 0, 0, 0, 0), (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream);
-        if (rc) return fail(SDIM_EHIP, "plain init launch failed");
-      }
-      for (const Prim& p : jp.prims) {
-        DevOp o;
-        if (p.kind == OP_NOISE) o = mkop(OP_NOISE, p.a, 0, p.q, p.aux, p.src);
-        else if (p.kind == OP_MEAS || p.kind == OP_RESET) o = mkop(p.kind, 0, 0, p.q, p.aux, p.src);
-        else o = mkop(p.kind, p.a, p.b, p.q, p.t, p.src);
-        rc = launch_plain_op(Q, o, (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream);
-        if (rc) return fail(SDIM_EHIP, "plain op launch failed");
-      }
-    }
-  }
+  rc = enqueue(T.get(), 0);
+  if (rc) return rc;
   HIP_OK(hipEventRecord(T->ev1, T->stream));
   // records back to the host (async; valid after wait)
   T->h_rec.resize(T->rec_bytes);
@@ -692,6 +726,7 @@ int sdim_release(sdim_ctx* c, uint64_t ticket) {
   int rc = finish(c, t);
   ctx_free(c, t->arena);
   ctx_free(c, t->meta);
+  for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
   c->tickets.erase(it);
@@ -777,6 +812,45 @@ int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t*
   if (bytes_lds) *bytes_lds = t->h_counters[1];
   if (n_items) *n_items = (int64_t)t->items.size();
   if (n_lds_items) *n_lds_items = t->n_lds_items;
+  return 0;
+}
+
+int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms) {
+  if (!c || iters < 1 || which < 0 || which > 2) return fail(SDIM_EINVAL, "bad replay arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  for (auto& it : t->items)
+    if (it.flags & ITEM_INIT_GIVEN) return fail(SDIM_EINVAL, "replay of a job with an initial tableau is not supported");
+  HIP_OK(hipSetDevice(c->device));
+  if (t->mode == SDIM_MODE_BATCHED && !t->graph[which]) {
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(t, which);
+    hipError_t e = hipStreamEndCapture(t->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&t->graph[which], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  HIP_OK(hipEventRecord(t->ev0, t->stream));
+  for (int32_t i = 0; i < iters; i++) {
+    if (t->mode == SDIM_MODE_BATCHED) {
+      HIP_OK(hipGraphLaunch(t->graph[which], t->stream));
+    } else {
+      rc = enqueue(t, which);
+      if (rc) return rc;
+    }
+  }
+  HIP_OK(hipEventRecord(t->ev1, t->stream));
+  HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, 16, hipMemcpyDeviceToHost, t->stream));
+  HIP_OK(hipStreamSynchronize(t->stream));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
   return 0;
 }
 

@@ -116,6 +116,13 @@ class Ticket:
         return {"kernel_ms": ms.value, "bytes_hbm": bh.value, "bytes_lds": bl.value,
                 "items": ni.value, "lds_items": nl.value}
 
+    def replay(self, iters=1, which="all"):
+        """Re-run the device work `iters` times; returns device milliseconds for all iterations."""
+        w = {"all": 0, "hbm": 1, "lds": 2}[which]
+        ms = ctypes.c_double(0)
+        _lib.check(self.engine.L.sdim_replay(self.engine.ctx, self.id, int(iters), w, ctypes.byref(ms)))
+        return ms.value
+
     def release(self):
         if not self._released:
             self._released = True

