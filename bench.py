@@ -43,7 +43,23 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--shots-per-item", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB) to calibrate "
+                         "rocprof FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    nbytes = 1 << 30
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.bitwise_or(x, 0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                                           "launches": 3, "kernel": "torch bitwise_or(int32, 0)"}}), flush=True)
 
 
 def make_workload(name):
@@ -84,6 +100,9 @@ def cpu_baseline(jobs, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
                                    "(sharding is described in DESIGN.md, not built)"}))

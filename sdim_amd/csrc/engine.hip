@@ -430,8 +430,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       const uint32_t lim = n - 64 * w;
       if (lim < 64) v &= (1ull << lim) - 1;
       if (v) atomicMin(&S.misc[0], (int)(64 * w + __builtin_ctzll(v)));
+      bytes += 8;
     }
-    bytes += 8 * nw;
   } else {
     const uint32_t* xr = row32(T.X, q, rs);
     const uint32_t nw = (n + 3) / 4;
@@ -440,8 +440,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       const uint32_t lim = n - 4 * w;
       if (lim < 4) v &= (1u << (8 * lim)) - 1;
       if (v) atomicMin(&S.misc[0], (int)(4 * w + (__builtin_ctz(v) >> 3)));
+      bytes += 4;
     }
-    bytes += 4 * nw;
   }
   __syncthreads();
   const int piv = S.misc[0];
@@ -457,8 +457,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         S.xp[r] = T.X[(size_t)r * rs + p];
         S.zp[r] = T.Z[(size_t)r * rs + p];
       }
+      bytes += (D == 2 ? 16 : 2);
     }
-    bytes += 2ull * n * (D == 2 ? 8 : 1);
     __syncthreads();
     uint32_t xz = 0;
     if constexpr (D != 2) {
@@ -500,8 +500,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         if (((n + p) >> 2) == w) f &= ~(0xFFu << (8 * ((n + p) & 3)));
         ((uint32_t*)S.F)[w] = f;
       }
+      bytes += (D == 2 ? 8 : 4);
     }
-    bytes += (D == 2 ? 8 : 4) * NWd;
     for (uint32_t s = tid; s < Sh; s += NT) {
       if constexpr (D == 2) {
         const uint32_t b0 = (plane(T, s, 0)[p >> 6] >> (p & 63)) & 1;
@@ -763,8 +763,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         const uint32_t f = T.X[(size_t)q * rs + n + i];
         S.F[i] = (uint8_t)f;
         S.C[i] = (uint8_t)((f * (f - 1) / 2) % D);
+        bytes += 1;
       }
-      bytes += n;
       __syncthreads();
       // per row: exclusive scan along the stabilizer columns, one wave per row
       const uint32_t lane = tid & 63, wid = tid >> 6;
@@ -850,7 +850,7 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   } else {
     init_identity<D, NT>(T, T_bytes);
   }
-  bytes += T_bytes;
+  if (threadIdx.x == 0) bytes += T_bytes;
 
   const DevOp* ops = P.ops + it.ops_off;
   uint32_t k = 0;
@@ -869,6 +869,7 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   __syncthreads();
   if (res) {
     copy16<NT>(gtab, lds, T_bytes);
+    if (threadIdx.x == 0) bytes += T_bytes;
     __syncthreads();
   }
   const unsigned long long tot = block_sum64<NT>(bytes, S.red);

@@ -847,6 +847,8 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
   }
   HIP_OK(hipEventRecord(t->ev1, t->stream));
   HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, 16, hipMemcpyDeviceToHost, t->stream));
+  if (t->rec_bytes)
+    HIP_OK(hipMemcpyAsync(t->h_rec.data(), t->arena.p, t->rec_bytes, hipMemcpyDeviceToHost, t->stream));
   HIP_OK(hipStreamSynchronize(t->stream));
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));

@@ -133,8 +133,9 @@ def test_replay_reproduces_records():
     jobs = [clifford_job(rng, n, d, 3 * n, shots=4, stream=i) for i, (n, d) in enumerate([(30, 2), (60, 3), (220, 5)])]
     tk = eng.run(jobs)
     before = [tk.records(i).copy() for i in range(len(jobs))]
-    tk.replay(2)
-    for i in range(len(jobs)):
-        assert np.array_equal(tk.records(i), before[i]) or True  # records buffer is host-side after submit
+    for which in ("all", "hbm", "lds", "all"):
+        tk.replay(2, which)
+        for i in range(len(jobs)):
+            assert np.array_equal(tk.records(i), before[i])
     check_ticket(tk, jobs)
     tk.release()
