@@ -75,12 +75,14 @@ const char* sdim_last_error(void);
 
 /*
  * Tuning / engine options (no reference counterpart).  Keys:
- *   "threads"         workgroup size of the batch kernel: 256, 512 or 1024
- *   "lds_bytes"       LDS budget per workgroup (<= 163824)
+ *   "threads"         workgroup size of the batch kernel: 0 (per launch class), 256, 512 or 1024
+ *   "lds_bytes"       upper bound of the LDS budget per workgroup (<= 163584)
  *   "shots_per_item"  shots that share one X/Z block per work item (0 = automatic, 1 = a
  *                     separate tableau per shot)
  *   "grid"            persistent grid size (0 = automatic)
  *   "allow_lds"       0 forces every tableau to stay in HBM
+ *   "hbm_threads"     workgroup size of the launch class for small HBM-resident items (64..1024)
+ *   "hbm_per_cu"      workgroups per CU of that class (its LDS budget is 160 KB / hbm_per_cu)
  */
 int sdim_set_option(sdim_ctx* ctx, const char* key, int64_t value);
 int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
@@ -121,6 +123,14 @@ int sdim_fetch_tableau(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t shot
  */
 int sdim_ticket_stats(sdim_ctx* ctx, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm,
                       uint64_t* bytes_lds, int64_t* n_items, int64_t* n_lds_items);
+
+/*
+ * Raw device counters of the last run of a ticket: [0] tableau bytes of HBM-resident
+ * items, [1] of LDS-resident items, then per-phase clock ticks when the library is
+ * built with SDIM_PHASE_TIMERS (zero otherwise).  Returns the number of values written.
+ * Diagnostics only; no reference counterpart.
+ */
+int sdim_ticket_counters(sdim_ctx* ctx, uint64_t ticket, uint64_t* out, int32_t count);
 
 /*
  * Re-run the device work of a finished ticket `iters` times back to back, with every

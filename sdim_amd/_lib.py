@@ -14,6 +14,7 @@ EXPORTS = (
     "sdim_ctx_create", "sdim_ctx_destroy", "sdim_last_error", "sdim_set_option", "sdim_get_option",
     "sdim_submit", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
     "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats", "sdim_replay",
+    "sdim_ticket_counters",
 )
 
 MODE_BATCHED = 0
@@ -68,10 +69,12 @@ def load(path=None):
     L.sdim_ticket_stats.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                     ctypes.POINTER(u64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.sdim_replay.argtypes = [vp, u64, i32, i32, ctypes.POINTER(ctypes.c_double)]
+    L.sdim_ticket_counters.argtypes = [vp, u64, ctypes.POINTER(u64), i32]
     for name in EXPORTS:
         f = getattr(L, name)
         if name != "sdim_last_error":
             f.restype = ctypes.c_int
+    L.sdim_ticket_counters.restype = ctypes.c_int
     if path is None:
         _lib = L
     return L

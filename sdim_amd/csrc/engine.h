@@ -6,6 +6,9 @@
 
 namespace sdim {
 
+// number of optional phase timers after the two byte counters
+#define SDIM_NUM_TIMERS 12
+
 struct KParams {
   const ItemDesc* items;
   const uint32_t* order;
@@ -19,12 +22,15 @@ struct KParams {
 };
 
 // LDS scratch of one workgroup for an item with n qudits, row stride rs, S shots
+// (must match carve() in engine.hip)
 __host__ __device__ inline uint32_t scratch_bytes(uint32_t n, uint32_t rs, uint32_t S, uint32_t NT) {
-  uint32_t b = 256 + 16 * 16;
-  b += 2 * ((n + 15u) & ~15u);
-  b += 4 * rs;          // F, C, DW (two planes)
-  b += NT * 16;
-  b += ((2 * S * 4) + 15u) & ~15u;
+  uint32_t b = 256 + 16 * 16;                 // counters, multiplication tables
+  b += 2 * ((n + 15u) & ~15u);                // pivot column X, Z
+  b += (2 * n + 15u) & ~15u;                  // touched rows
+  b += ((rs / 2) + 15u) & ~15u;               // words with f != 0
+  b += 6 * rs;                                // row q, f, dots (2 rs), segment delta (2 rs)
+  b += NT * 16;                               // reductions
+  b += ((2 * S * 4) + 15u) & ~15u;            // per-shot pivot phase / outcome
   return b;
 }
 

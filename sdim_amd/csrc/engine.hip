@@ -12,20 +12,29 @@
 // Execution model.  An item is one circuit with S shots; its X/Z blocks are
 // shared by the shots and each shot owns a phase vector (sdim_types.h).  One
 // workgroup processes one item at a time.  The circuit arrives as segments of
-// gates (already layered: the gates of a layer touch disjoint qudits) and
-// measurement ops:
-//   * a gate segment is processed layer by layer; thread (word w, slot k)
-//     applies the k-th, (k+NS)-th, ... gate of the layer to column word w.
-//     The phase increments of gates do not depend on the shot, so each thread
-//     accumulates them in registers (delta) and the per-shot phase vectors are
-//     touched once at the end of the segment.  Noise ops snapshot their row;
-//     their shot-dependent phase terms are added in the same end-of-segment pass.
-//   * a measurement is a cooperative workgroup operation: ballot/atomic-min
-//     pivot search over the stabilizer half of row q, gather of the pivot
-//     column, a single sweep over all rows that applies the rank-1 update
-//     X += x_p f^T, Z += z_p f^T and accumulates the column dots <z_i, x_p>
-//     from the old Z, then per-shot phase updates; or, without a pivot, the
-//     deterministic ancilla as a per-row exclusive scan.
+// gates (layered on the host: the gates of a layer touch disjoint qudits) and
+// measurement ops.
+//   * Gate segment: layer by layer, thread (word w, slot k) applies the k-th,
+//     (k+NS)-th, ... gate of the layer to column word w, loading the rows of a
+//     batch of independent gates before computing (latency hiding).  Gate phase
+//     increments do not depend on the shot, so each thread sums them in
+//     registers (delta) and the shots' phase vectors are touched once, at the
+//     end of the segment.  Noise ops snapshot their row; their shot-dependent
+//     phase terms are added in the same end-of-segment pass.
+//   * Measurement: four dependent steps separated by barriers.
+//       A  read row q: pivot candidate (atomic-min over the stabilizer half),
+//          rowsum factors f = -x[q, .], list of words with f != 0
+//       B  gather pivot column p: list of touched rows (x_p or z_p != 0),
+//          <x_p, z_p>, pivot phases of every shot
+//       C  sweep (touched row) x (word with f != 0): X += x_p f, Z += z_p f with
+//          x_p, z_p scaled by e = x[q,p]^-1 on the fly (exponentiation), column
+//          dots <z_i, x_p> from the old Z accumulated through LDS atomics
+//       D  columns p / n+p rewritten (stabilizer p <- Z_q, destabilizer p <- old
+//          stabilizer p), per-shot phases ph += f ph_p + f dot + xz f(f-1)/2
+//     then per-shot outcomes.  Without a pivot the outcome is the deterministic
+//     ancilla, computed per row as an exclusive scan over the columns with f != 0.
+//   The work of a rowsum is (rows in the support of the pivot generator) x
+//   (words where f != 0), which is what the sweep iterates over.
 // The tableau lives in LDS for the whole item when it fits (ITEM_LDS) and in
 // HBM otherwise; the same code runs on both (address spaces are inferred
 // after inlining).
@@ -41,19 +50,22 @@ namespace sdim {
 
 // ------------------------------------------------------------------ scratch carve
 struct Scr {
-  int* misc;         // 64 ints
-  MulTab* mtab;      // 16 tables
-  uint8_t* xp;       // pivot column X, n bytes
+  int* misc;         // two counter sets of 16 ints (double-buffered per measurement)
+  MulTab* mtab;      // tables of c * v mod d, c < 16
+  uint8_t* xp;       // pivot column X, n bytes (0/1 for d = 2)
   uint8_t* zp;       // pivot column Z, n bytes
-  uint8_t* F;        // rs bytes: rowsum factors f (odd: bytes; d=2: bits)
-  uint8_t* C;        // rs bytes: shot-invariant phase term per column
-  uint8_t* DW;       // 2*rs bytes: segment phase delta (d = 2 uses two planes)
+  uint16_t* TR;      // touched rows of a rowsum
+  uint16_t* FL;      // words with f != 0
+  uint8_t* RQ;       // row q of X, rs bytes
+  uint8_t* F;        // rowsum factors, rs bytes
+  uint32_t* DOT;     // column dots: two u32 per word
+  uint8_t* DW;       // segment phase delta: 2 * rs bytes
   uint32_t* red;     // NT * 4 words
   uint32_t* shot;    // 2 * S words
   uint8_t* noise;    // noise tables of a segment
+  uint8_t* lds_end;  // end of this workgroup's dynamic LDS
+  bool tiling;       // stage column tiles of an HBM-resident tableau through LDS for gate segments
 };
-
-SDIM_DEV uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
 template <int NT>
 SDIM_DEV Scr carve(uint8_t* base, uint32_t n, uint32_t rs, uint32_t S) {
@@ -61,22 +73,22 @@ SDIM_DEV Scr carve(uint8_t* base, uint32_t n, uint32_t rs, uint32_t S) {
   uint8_t* p = base;
   s.misc = (int*)p; p += 256;
   s.mtab = (MulTab*)p; p += 16 * 16;
-  uint32_t nb = (n + 15u) & ~15u;
+  const uint32_t nb = (n + 15u) & ~15u;
   s.xp = p; p += nb;
   s.zp = p; p += nb;
+  s.TR = (uint16_t*)p; p += (2 * n + 15u) & ~15u;
+  s.FL = (uint16_t*)p; p += ((rs / 2) + 15u) & ~15u;
+  s.RQ = p; p += rs;
   s.F = p; p += rs;
-  s.C = p; p += rs;
+  s.DOT = (uint32_t*)p; p += 2 * rs;
   s.DW = p; p += 2 * rs;
   s.red = (uint32_t*)p; p += NT * 16;
   s.shot = (uint32_t*)p; p += ((2 * S * 4) + 15u) & ~15u;
   s.noise = p;
+  s.lds_end = nullptr;
+  s.tiling = false;
   return s;
 }
-
-template <int D> struct Cfg {
-  static constexpr int PO = (D == 2) ? 2 : 1;
-  static constexpr int ORD = (D == 2) ? 4 : D;
-};
 
 // tableau pointers of an item
 struct Tab {
@@ -84,23 +96,36 @@ struct Tab {
   uint8_t* Z;
   uint8_t* Ph;
   uint32_t n, rs, S;
+  uint32_t wofs;      // first column word of this view (column tiles of an HBM-resident tableau)
+  uint32_t rs_full;   // row stride of the whole tableau (noise snapshots are indexed by it)
 };
 
 SDIM_DEV uint32_t* row32(uint8_t* base, uint32_t r, uint32_t rs) { return (uint32_t*)(base + (size_t)r * rs); }
 SDIM_DEV uint64_t* row64(uint8_t* base, uint32_t r, uint32_t rs) { return (uint64_t*)(base + (size_t)r * rs); }
-
-// per-shot phase vector (odd d) / bit-plane (d = 2)
 SDIM_DEV uint8_t* phase_vec(const Tab& T, uint32_t s) { return T.Ph + (size_t)s * T.rs; }
 SDIM_DEV uint64_t* plane(const Tab& T, uint32_t s, int k) { return (uint64_t*)(T.Ph + (size_t)(2 * s + k) * T.rs); }
+
+// Optional per-phase timers (build with -DSDIM_PHASE_TIMERS): thread 0 of each
+// workgroup accumulates wall-clock ticks per phase into counters[2 + phase].
+#ifdef SDIM_PHASE_TIMERS
+#define SDIM_T0() const uint64_t t_start_ = (threadIdx.x == 0) ? wall_clock64() : 0
+#define SDIM_TMARK(var) uint64_t var = (threadIdx.x == 0) ? wall_clock64() : 0
+#define SDIM_TADD(P, ph, from) do { if (threadIdx.x == 0) atomicAdd(&(P).counters[2 + (ph)], (unsigned long long)(wall_clock64() - (from))); } while (0)
+#else
+#define SDIM_T0()
+#define SDIM_TMARK(var)
+#define SDIM_TADD(P, ph, from) do {} while (0)
+#endif
+
+// fold the two 16-bit partial sums of a 32-bit word mod D
+template <int D> SDIM_DEV uint32_t fold16(uint32_t v) { return ((v & 0xFFFFu) % D) | (((v >> 16) % D) << 16); }
 
 // ------------------------------------------------------------------ block reductions
 template <int NT>
 SDIM_DEV uint32_t block_sum(uint32_t v, uint32_t* red) {
-  // wave reduction then across waves
   #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
   if (lane == 0) red[wid] = v;
   __syncthreads();
   uint32_t r = 0;
@@ -136,7 +161,7 @@ SDIM_DEV void init_identity(const Tab& T, uint64_t total) {
   const uint32_t n = T.n;
   for (uint32_t r = threadIdx.x; r < n; r += NT) {
     if constexpr (D == 2) {
-      uint32_t g = n + r;
+      const uint32_t g = n + r;
       row64(T.X, r, T.rs)[g >> 6] |= 1ull << (g & 63);
       row64(T.Z, r, T.rs)[r >> 6] |= 1ull << (r & 63);
     } else {
@@ -154,132 +179,113 @@ SDIM_DEV void copy16(uint8_t* dst, const uint8_t* src, uint64_t bytes) {
   for (uint64_t i = threadIdx.x; i < bytes / 16; i += NT) d[i] = s[i];
 }
 
-// ------------------------------------------------------------------ gate on one column word
-// odd d: one 32-bit word = 4 generators.  Returns bytes touched.
+// ------------------------------------------------------------------ one gate on one column word
+// The caller has loaded the row words (xq, zq of qudit q; xt, zt of the target
+// for SUM).  Stores the updated words and accumulates the phase increment.
 template <int D>
-SDIM_DEV uint32_t gate_odd(uint32_t kind, uint32_t q, uint32_t t, uint32_t a, uint32_t b,
-                           const Tab& T, uint32_t w, uint32_t& dl, const Scr& S, uint32_t nslot) {
+SDIM_DEV uint32_t apply_odd(const DevOp& o, uint32_t xq, uint32_t zq, uint32_t xt, uint32_t zt,
+                            const Tab& T, uint32_t w, uint32_t& dl, const Scr& S) {
   const uint32_t rs = T.rs;
-  uint32_t* xq = row32(T.X, q, rs) + w;
-  uint32_t* zq = row32(T.Z, q, rs) + w;
-  switch (kind) {
-    case OP_H: {
-      uint32_t x = *xq, z = *zq;
-      *xq = negm<D>(z); *zq = x;
-      dl = subm<D>(dl, mulv<D>(x, z));   // phase += po * (-z) * x
+  uint32_t* pxq = row32(T.X, o.q, rs) + w;
+  uint32_t* pzq = row32(T.Z, o.q, rs) + w;
+  switch (op_kind(o)) {
+    case OP_H:                                     // (x, z) <- (-z, x); phase += po (-z) x
+      *pxq = negm<D>(zq); *pzq = xq;
+      dl = subm<D>(dl, mulv<D>(xq, zq));
       return 16;
-    }
-    case OP_HI: {
-      uint32_t x = *xq, z = *zq;
-      *xq = z; *zq = negm<D>(x);
-      dl = subm<D>(dl, mulv<D>(x, z));   // phase += po * z * (-x)
+    case OP_HI:                                    // (x, z) <- (z, -x); phase += po z (-x)
+      *pxq = zq; *pzq = negm<D>(xq);
+      dl = subm<D>(dl, mulv<D>(xq, zq));
       return 16;
-    }
-    case OP_P: {
+    case OP_P: {                                   // z += x; phase += x (x - 1) / 2
       const MulTab h = make_half_tab<D>();
-      uint32_t x = *xq, z = *zq;
-      *zq = addm<D>(z, x);
-      dl = addm<D>(dl, mult<D>(h, x));   // phase += x(x-1)/2
+      *pzq = addm<D>(zq, xq);
+      dl = addm<D>(dl, mult<D>(h, xq));
       return 12;
     }
     case OP_PI: {
       const MulTab h = make_half_tab<D>();
-      uint32_t x = *xq, z = *zq;
-      *zq = subm<D>(z, x);
-      dl = subm<D>(dl, mult<D>(h, x));
+      *pzq = subm<D>(zq, xq);
+      dl = subm<D>(dl, mult<D>(h, xq));
       return 12;
     }
-    case OP_SUM: {                       // x[t] += x[c]; z[c] += (d-1) z[t]
-      uint32_t* xt = row32(T.X, t, rs) + w;
-      uint32_t* zt = row32(T.Z, t, rs) + w;
-      *xt = addm<D>(*xt, *xq);
-      *zq = subm<D>(*zq, *zt);
+    case OP_SUM:                                   // x[t] += x[c]; z[c] += (d - 1) z[t]
+      *(row32(T.X, o.t, rs) + w) = addm<D>(xt, xq);
+      *pzq = subm<D>(zq, zt);
       return 24;
-    }
-    case OP_SUMI: {
-      uint32_t* xt = row32(T.X, t, rs) + w;
-      uint32_t* zt = row32(T.Z, t, rs) + w;
-      *xt = subm<D>(*xt, *xq);
-      *zq = addm<D>(*zq, *zt);
+    case OP_SUMI:
+      *(row32(T.X, o.t, rs) + w) = subm<D>(xt, xq);
+      *pzq = addm<D>(zq, zt);
       return 24;
-    }
-    case OP_PAULI: {                     // X^a Z^b: phase += b x - a z
-      uint32_t x = *xq, z = *zq;
-      dl = addm<D>(dl, subm<D>(mult<D>(S.mtab[b], x), mult<D>(S.mtab[a], z)));
+    case OP_PAULI:                                 // X^a Z^b: phase += b x - a z
+      dl = addm<D>(dl, subm<D>(mult<D>(S.mtab[op_b(o)], xq), mult<D>(S.mtab[op_a(o)], zq)));
       return 8;
-    }
-    case OP_NOISE: {                     // snapshot of row q for the end-of-segment pass
-      uint8_t* snap = S.noise + nslot * 2 * rs;
-      ((uint32_t*)snap)[w] = *xq;
-      ((uint32_t*)(snap + rs))[w] = *zq;
+    case OP_NOISE: {                               // snapshot of row q for the end-of-segment pass
+      uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
+      ((uint32_t*)snap)[T.wofs + w] = xq;
+      ((uint32_t*)(snap + T.rs_full))[T.wofs + w] = zq;
       return 8;
     }
   }
   return 0;
 }
 
-// d = 2: one 64-bit word = 64 generators; delta is the plane pair (d0, d1).
-SDIM_DEV uint32_t gate_bin(uint32_t kind, uint32_t q, uint32_t t, uint32_t a, uint32_t b,
-                           const Tab& T, uint32_t w, uint64_t& d0, uint64_t& d1, const Scr& S, uint32_t nslot) {
+SDIM_DEV uint32_t apply_bin(const DevOp& o, uint64_t xq, uint64_t zq, uint64_t xt, uint64_t zt,
+                            const Tab& T, uint32_t w, uint64_t& d0, uint64_t& d1, const Scr& S) {
   const uint32_t rs = T.rs;
-  uint64_t* xq = row64(T.X, q, rs) + w;
-  uint64_t* zq = row64(T.Z, q, rs) + w;
-  switch (kind) {
+  uint64_t* pxq = row64(T.X, o.q, rs) + w;
+  uint64_t* pzq = row64(T.Z, o.q, rs) + w;
+  switch (op_kind(o)) {
     case OP_H:
-    case OP_HI: {                        // (x, z) <- (z, x); phase += 2 x z
-      uint64_t x = *xq, z = *zq;
-      *xq = z; *zq = x;
-      d1 ^= x & z;
+    case OP_HI:                                    // swap; phase += 2 x z
+      *pxq = zq; *pzq = xq;
+      d1 ^= xq & zq;
       return 32;
-    }
-    case OP_P: {                         // phase += x; z ^= x
-      uint64_t x = *xq, z = *zq;
-      *zq = z ^ x;
-      d1 ^= d0 & x; d0 ^= x;
+    case OP_P:                                     // phase += x; z ^= x
+      *pzq = zq ^ xq;
+      d1 ^= d0 & xq; d0 ^= xq;
       return 24;
-    }
-    case OP_PI: {                        // phase += 3 x; z ^= x
-      uint64_t x = *xq, z = *zq;
-      *zq = z ^ x;
-      d1 ^= x & ~d0; d0 ^= x;
+    case OP_PI:                                    // phase += 3 x; z ^= x
+      *pzq = zq ^ xq;
+      d1 ^= xq & ~d0; d0 ^= xq;
       return 24;
-    }
     case OP_SUM:
-    case OP_SUMI: {
-      uint64_t* xt = row64(T.X, t, rs) + w;
-      uint64_t* zt = row64(T.Z, t, rs) + w;
-      *xt ^= *xq;
-      *zq ^= *zt;
+    case OP_SUMI:
+      *(row64(T.X, o.t, rs) + w) = xt ^ xq;
+      *pzq = zq ^ zt;
       return 48;
-    }
-    case OP_PAULI: {                     // phase += 2 (b x - a z)
-      uint64_t x = *xq, z = *zq;
-      d1 ^= ((b & 1) ? x : 0ull) ^ ((a & 1) ? z : 0ull);
+    case OP_PAULI:                                 // phase += 2 (b x - a z)
+      d1 ^= ((op_b(o) & 1) ? xq : 0ull) ^ ((op_a(o) & 1) ? zq : 0ull);
       return 16;
-    }
     case OP_NOISE: {
-      uint8_t* snap = S.noise + nslot * 2 * rs;
-      ((uint64_t*)snap)[w] = *xq;
-      ((uint64_t*)(snap + rs))[w] = *zq;
+      uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
+      ((uint64_t*)snap)[T.wofs + w] = xq;
+      ((uint64_t*)(snap + T.rs_full))[T.wofs + w] = zq;
       return 16;
     }
   }
   return 0;
 }
+
+SDIM_DEV bool two_rows(const DevOp& o) { return op_kind(o) == OP_SUM || op_kind(o) == OP_SUMI; }
 
 // ------------------------------------------------------------------ gate segment
 // ops[k] is the OP_SEG header.  Returns the index after the segment.
 template <int D, int NT>
 SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const Scr& S,
-                              const ItemDesc& it, const KParams& P, unsigned long long& bytes) {
+                              const ItemDesc& it, const KParams& P, unsigned long long& bytes,
+                              unsigned long long& lbytes) {
+  constexpr bool BIN = (D == 2);
+  constexpr int GB = 4;     // gates per load batch
+  SDIM_TMARK(tSeg);
   const DevOp hdr = ops[k];
   const uint32_t nl = hdr.q, nn = hdr.t;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t tid = threadIdx.x;
   const uint32_t G = 2 * n;
-  const uint32_t NWd = (D == 2) ? (G + 63) / 64 : (G + 3) / 4;
   const uint32_t ndesc = k + 1;
   k += 1 + nn;
+  unsigned long long gb = 0;                   // bytes of the gate updates
 
   // noise draws of the segment: mask[j][S/32] bits + ab[j][S] bytes
   const uint32_t mw = (Sh + 31) / 32;
@@ -291,408 +297,479 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
     for (uint32_t i = tid; i < nn * Sh; i += NT) {
       const uint32_t j = i / Sh, s = i - j * Sh;
       const DevOp nd = ops[ndesc + j];
-      U64x4 o = draw(it.seed, it.stream, nd.src, it.shot0 + s);
-      uint32_t ab = noise_ab(o, D, op_a(nd), P.keep53[it.keep_off + nd.t]);
+      const U64x4 o = draw(it.seed, it.stream, nd.src, it.shot0 + s);
+      const uint32_t ab = noise_ab(o, D, op_a(nd), P.keep53[it.keep_off + nd.t]);
       nab[i] = (uint8_t)(ab & 0xFF);
       if (ab) atomicOr(&nmask[j * mw + (s >> 5)], 1u << (s & 31));
     }
     __syncthreads();
   }
 
-  // word ownership
+  // Column tiles: an HBM-resident tableau is staged through LDS, TW bytes of every
+  // row at a time (gates act on each generator column independently, so the tiles
+  // need no coordination); an LDS-resident tableau is a single tile.
+  uint32_t TW = rs;
+  uint8_t* tile = nullptr;
+  if (S.tiling) {
+    tile = S.noise + ((noise_bytes(nn, rs, Sh) + 15u) & ~15u);
+    const int64_t left = (int64_t)(S.lds_end - tile);
+    const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
+    uint64_t tw = room / (2ull * n);
+    tw &= ~15ull;
+    TW = tw >= 16 ? (uint32_t)(tw < rs ? tw : rs) : 0;
+  }
+  const uint32_t CPWb = BIN ? 8 : 4;            // bytes per column word
+  for (uint32_t c0 = 0; c0 < rs; c0 += (TW ? TW : rs)) {
+    Tab V = T;
+    uint32_t vw = rs;                            // bytes of each row in this view
+    if (S.tiling && TW) {
+      vw = (rs - c0 < TW) ? rs - c0 : TW;
+      V.X = tile; V.Z = tile + (size_t)n * vw; V.rs = vw; V.wofs = c0 / CPWb;
+      // load the tile (16-byte chunks)
+      const uint32_t cpr = vw / 16;
+      for (uint32_t i = tid; i < 2 * n * cpr; i += NT) {
+        const uint32_t rr = i / cpr, cc = i - rr * cpr;
+        const uint8_t* src = (rr < n ? T.X + (size_t)rr * rs : T.Z + (size_t)(rr - n) * rs) + c0 + 16 * cc;
+        *(uint4*)(tile + (size_t)rr * vw + 16 * cc) = *(const uint4*)src;
+        bytes += 16;
+      }
+      __syncthreads();
+    }
+    const uint32_t NWd = vw / CPWb;
+    uint32_t kl = k;
+  // word ownership: (word, slot) pairs when a row has at most NT words (the
+  // slot partial deltas are reduced at the end); otherwise each thread owns
+  // several whole words and adds each (layer, word) delta into DW directly.
   uint32_t NS, my_w, my_slot;
-  if (NWd <= NT) { NS = NT / NWd; my_w = tid % NWd; my_slot = tid / NWd; }
+  const bool many = NWd > NT;
+  if (!many) { NS = NT / NWd; my_w = tid % NWd; my_slot = tid / NWd; }
   else { NS = 1; my_w = tid; my_slot = 0; }
   const bool active = my_slot < NS;
-  constexpr int KW = 4;  // words per thread supported when NWd > NT
-  uint32_t dl[KW];
-  uint64_t b0[KW], b1[KW];
-  #pragma unroll
-  for (int i = 0; i < KW; i++) { dl[i] = 0; b0[i] = 0; b1[i] = 0; }
+  if (many) {
+    for (uint32_t w = tid; w < NWd; w += NT) {
+      if constexpr (BIN) { ((uint64_t*)S.DW)[V.wofs + w] = 0; ((uint64_t*)(S.DW + rs))[V.wofs + w] = 0; }
+      else ((uint32_t*)S.DW)[V.wofs + w] = 0;
+    }
+  }
+  uint32_t dl = 0;
+  uint64_t b0 = 0, b1 = 0;
 
   for (uint32_t l = 0; l < nl; l++) {
-    const uint32_t L = ops[k].q;
-    const DevOp* lops = ops + k + 1;
+    const uint32_t L = ops[kl].q;
+    const DevOp* lops = ops + kl + 1;
     if (active) {
-      #pragma unroll
-      for (int kw = 0; kw < KW; kw++) {
-        const uint32_t w = my_w + (uint32_t)kw * NT;
-        if (w < NWd && (kw == 0 || NWd > NT)) {
-          for (uint32_t g = my_slot; g < L; g += NS) {
-            const DevOp o = lops[g];
-            if constexpr (D == 2)
-              bytes += gate_bin(op_kind(o), o.q, o.t, op_a(o), op_b(o), T, w, b0[kw], b1[kw], S, op_noise_slot(o));
-            else
-              bytes += gate_odd<D>(op_kind(o), o.q, o.t, op_a(o), op_b(o), T, w, dl[kw], S, op_noise_slot(o));
+      for (uint32_t w = my_w; w < NWd; w += NT) {
+        if (many) { dl = 0; b0 = 0; b1 = 0; }
+        for (uint32_t g0 = my_slot; g0 < L; g0 += NS * GB) {
+          DevOp ob[GB];
+          if constexpr (BIN) {
+            uint64_t xq[GB], zq[GB], xt[GB], zt[GB];
+            #pragma unroll
+            for (int b = 0; b < GB; b++) {
+              const uint32_t g = g0 + (uint32_t)b * NS;
+              if (g < L) {
+                ob[b] = lops[g];
+                xq[b] = row64(V.X, ob[b].q, V.rs)[w];
+                zq[b] = row64(V.Z, ob[b].q, V.rs)[w];
+                if (two_rows(ob[b])) { xt[b] = row64(V.X, ob[b].t, V.rs)[w]; zt[b] = row64(V.Z, ob[b].t, V.rs)[w]; }
+                else { xt[b] = 0; zt[b] = 0; }
+              }
+            }
+            #pragma unroll
+            for (int b = 0; b < GB; b++)
+              if (g0 + (uint32_t)b * NS < L) gb += apply_bin(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, b0, b1, S);
+          } else {
+            uint32_t xq[GB], zq[GB], xt[GB], zt[GB];
+            #pragma unroll
+            for (int b = 0; b < GB; b++) {
+              const uint32_t g = g0 + (uint32_t)b * NS;
+              if (g < L) {
+                ob[b] = lops[g];
+                xq[b] = row32(V.X, ob[b].q, V.rs)[w];
+                zq[b] = row32(V.Z, ob[b].q, V.rs)[w];
+                if (two_rows(ob[b])) { xt[b] = row32(V.X, ob[b].t, V.rs)[w]; zt[b] = row32(V.Z, ob[b].t, V.rs)[w]; }
+                else { xt[b] = 0; zt[b] = 0; }
+              }
+            }
+            #pragma unroll
+            for (int b = 0; b < GB; b++)
+              if (g0 + (uint32_t)b * NS < L) gb += apply_odd<D>(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, dl, S);
           }
         }
+        if (many) {
+          if constexpr (BIN) {
+            uint64_t* p0 = (uint64_t*)S.DW + V.wofs + w;
+            uint64_t* p1 = (uint64_t*)(S.DW + rs) + V.wofs + w;
+            const uint64_t a0 = *p0, a1 = *p1;
+            *p1 = a1 ^ b1 ^ (a0 & b0);
+            *p0 = a0 ^ b0;
+          } else {
+            uint32_t* pd = (uint32_t*)S.DW + V.wofs + w;
+            *pd = addm<D>(*pd, dl);
+          }
+        }
+        if (!many) break;
       }
     }
-    k += 1 + L;
+    kl += 1 + L;
     __syncthreads();
   }
 
-  // reduce the deltas over slots into DW
-  if (NWd <= NT) {
-    if constexpr (D == 2) {
-      uint64_t* r = (uint64_t*)S.red;   // NT * 16 bytes: two words per thread
-      if (active) { r[2 * tid] = b0[0]; r[2 * tid + 1] = b1[0]; }
+  // reduce the slot deltas into DW
+  if (!many) {
+    if constexpr (BIN) {
+      uint64_t* r = (uint64_t*)S.red;
+      if (active) { r[2 * tid] = b0; r[2 * tid + 1] = b1; }
       __syncthreads();
       for (uint32_t w = tid; w < NWd; w += NT) {
         uint64_t a0 = 0, a1 = 0;
         for (uint32_t s = 0; s < NS; s++) {
           const uint32_t src = s * NWd + w;
-          uint64_t c0 = r[2 * src], c1 = r[2 * src + 1];
+          const uint64_t c0 = r[2 * src], c1 = r[2 * src + 1];
           a1 ^= c1 ^ (a0 & c0); a0 ^= c0;
         }
-        ((uint64_t*)S.DW)[w] = a0;
-        ((uint64_t*)(S.DW + rs))[w] = a1;
+        ((uint64_t*)S.DW)[V.wofs + w] = a0;
+        ((uint64_t*)(S.DW + rs))[V.wofs + w] = a1;
       }
     } else {
-      if (active) S.red[tid] = dl[0];
+      if (active) S.red[tid] = dl;
       __syncthreads();
       for (uint32_t w = tid; w < NWd; w += NT) {
         uint32_t a = 0;
         for (uint32_t s = 0; s < NS; s++) a = addm<D>(a, S.red[s * NWd + w]);
-        ((uint32_t*)S.DW)[w] = a;
-      }
-    }
-  } else {
-    #pragma unroll
-    for (int kw = 0; kw < KW; kw++) {
-      const uint32_t w = my_w + (uint32_t)kw * NT;
-      if (w < NWd) {
-        if constexpr (D == 2) { ((uint64_t*)S.DW)[w] = b0[kw]; ((uint64_t*)(S.DW + rs))[w] = b1[kw]; }
-        else ((uint32_t*)S.DW)[w] = dl[kw];
+        ((uint32_t*)S.DW)[V.wofs + w] = a;
       }
     }
   }
   __syncthreads();
 
+    if (S.tiling && TW) {
+      // write the tile back
+      const uint32_t cpr = vw / 16;
+      for (uint32_t i = tid; i < 2 * n * cpr; i += NT) {
+        const uint32_t rr = i / cpr, cc = i - rr * cpr;
+        uint8_t* dst = (rr < n ? T.X + (size_t)rr * rs : T.Z + (size_t)(rr - n) * rs) + c0 + 16 * cc;
+        *(uint4*)dst = *(const uint4*)(tile + (size_t)rr * vw + 16 * cc);
+        bytes += 16;
+      }
+      __syncthreads();
+    }
+    if (c0 + vw >= rs) { k = kl; break; }
+  }
+  // gate traffic lands in LDS when the tableau was staged through tiles
+  if (S.tiling && TW) lbytes += gb; else bytes += gb;
+  const uint32_t NWd = BIN ? (G + 63) / 64 : (G + 3) / 4;
+  SDIM_TADD(P, 7, tSeg);
+  SDIM_TMARK(tPh);
   // per-shot phase pass: Ph_s += delta + noise terms of shot s
   for (uint32_t i = tid; i < Sh * NWd; i += NT) {
     const uint32_t s = i / NWd, w = i - s * NWd;
-    if constexpr (D == 2) {
+    if constexpr (BIN) {
       uint64_t* p0 = plane(T, s, 0) + w;
       uint64_t* p1 = plane(T, s, 1) + w;
-      uint64_t c0 = ((uint64_t*)S.DW)[w], c1 = ((uint64_t*)(S.DW + rs))[w];
+      const uint64_t c0 = ((const uint64_t*)S.DW)[w];
+      uint64_t c1 = ((const uint64_t*)(S.DW + rs))[w];
       for (uint32_t j = 0; j < nn; j++) {
         if (nmask[j * mw + (s >> 5)] & (1u << (s & 31))) {
           const uint32_t ab = nab[j * Sh + s];
           const uint8_t* snap = S.noise + j * 2 * rs;
-          uint64_t sx = ((const uint64_t*)snap)[w], sz = ((const uint64_t*)(snap + rs))[w];
+          const uint64_t sx = ((const uint64_t*)snap)[w], sz = ((const uint64_t*)(snap + rs))[w];
           c1 ^= (((ab >> 4) & 1) ? sx : 0ull) ^ ((ab & 1) ? sz : 0ull);
         }
       }
-      uint64_t a0 = *p0, a1 = *p1;
-      *p1 = a1 ^ c1 ^ (a0 & c0);
-      *p0 = a0 ^ c0;
-      bytes += 32;
+      if (c0 | c1) {
+        const uint64_t a0 = *p0, a1 = *p1;
+        *p1 = a1 ^ c1 ^ (a0 & c0);
+        *p0 = a0 ^ c0;
+        bytes += 32;
+      }
     } else {
       uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
-      uint32_t c = ((uint32_t*)S.DW)[w];
+      uint32_t c = ((const uint32_t*)S.DW)[w];
       for (uint32_t j = 0; j < nn; j++) {
         if (nmask[j * mw + (s >> 5)] & (1u << (s & 31))) {
           const uint32_t ab = nab[j * Sh + s];
           const uint8_t* snap = S.noise + j * 2 * rs;
-          uint32_t sx = ((const uint32_t*)snap)[w], sz = ((const uint32_t*)(snap + rs))[w];
+          const uint32_t sx = ((const uint32_t*)snap)[w], sz = ((const uint32_t*)(snap + rs))[w];
           c = addm<D>(c, subm<D>(mult<D>(S.mtab[(ab >> 4) & 15], sx), mult<D>(S.mtab[ab & 15], sz)));
         }
       }
-      *ph = addm<D>(*ph, c);
-      bytes += 8;
+      if (c) {
+        *ph = addm<D>(*ph, c);
+        bytes += 8;
+      }
     }
   }
   __syncthreads();
+  SDIM_TADD(P, 8, tPh);
   return k;
 }
 
 // ------------------------------------------------------------------ measurement
 // Z-basis measurement of qudit q (tableau_prime.py:262-363) followed, for
 // RESET, by X^((-m) mod d) (program.py:334-339).  Writes one record per shot.
+// mk counts the measurements of the item (selects the counter set).
 template <int D, int NT>
-SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, bool reset,
+SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, bool reset, uint32_t mk,
                       const Scr& S, const ItemDesc& it, const KParams& P, unsigned long long& bytes) {
-  constexpr int PO = Cfg<D>::PO;
+  constexpr bool BIN = (D == 2);
+  constexpr uint32_t CPW = BIN ? 64 : 4;
+  constexpr int RB = 4;     // rows per load batch in the sweep
   const uint32_t tid = threadIdx.x;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t G = 2 * n;
+  const uint32_t NWd = (G + CPW - 1) / CPW;
+  const uint32_t nsw = (n + CPW - 1) / CPW;
+  int* M = S.misc + 16 * (mk & 1);
+  int* Mn = S.misc + 16 * ((mk + 1) & 1);
   uint8_t* rec = P.arena + it.rec_off + (size_t)slot * Sh;
+  SDIM_TMARK(tA);
 
-  // ---- pivot: first stabilizer column with x[q, i] != 0
-  if (tid == 0) S.misc[0] = INT_MAX;
-  __syncthreads();
-  if constexpr (D == 2) {
-    const uint64_t* xr = row64(T.X, q, rs);
-    const uint32_t nw = (n + 63) / 64;
-    for (uint32_t w = tid; w < nw; w += NT) {
-      uint64_t v = xr[w];
-      const uint32_t lim = n - 64 * w;
-      if (lim < 64) v &= (1ull << lim) - 1;
-      if (v) atomicMin(&S.misc[0], (int)(64 * w + __builtin_ctzll(v)));
+  // ---- A: row q -> pivot candidate, f words, list of words with f != 0
+  for (uint32_t w = tid; w < NWd; w += NT) {
+    S.DOT[2 * w] = 0;
+    S.DOT[2 * w + 1] = 0;
+    if constexpr (BIN) {
+      const uint64_t v = row64(T.X, q, rs)[w];
+      ((uint64_t*)S.RQ)[w] = v;
+      ((uint64_t*)S.F)[w] = v;                      // f = -x = x mod 2
+      if (v) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
+      if (w < nsw) {
+        uint64_t vs = v;
+        const uint32_t lim = n - 64 * w;
+        if (lim < 64) vs &= (1ull << lim) - 1;
+        if (vs) atomicMin(&M[0], (int)(64 * w + __builtin_ctzll(vs)));
+      }
       bytes += 8;
-    }
-  } else {
-    const uint32_t* xr = row32(T.X, q, rs);
-    const uint32_t nw = (n + 3) / 4;
-    for (uint32_t w = tid; w < nw; w += NT) {
-      uint32_t v = xr[w];
-      const uint32_t lim = n - 4 * w;
-      if (lim < 4) v &= (1u << (8 * lim)) - 1;
-      if (v) atomicMin(&S.misc[0], (int)(4 * w + (__builtin_ctz(v) >> 3)));
+    } else {
+      const uint32_t v = row32(T.X, q, rs)[w];
+      ((uint32_t*)S.RQ)[w] = v;
+      const uint32_t f = negm<D>(v);
+      ((uint32_t*)S.F)[w] = f;
+      if (f) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
+      if (w < nsw) {
+        uint32_t vs = v;
+        const uint32_t lim = n - 4 * w;
+        if (lim < 4) vs &= (1u << (8 * lim)) - 1;
+        if (vs) atomicMin(&M[0], (int)(4 * w + (__builtin_ctz(vs) >> 3)));
+      }
       bytes += 4;
     }
   }
-  __syncthreads();
-  const int piv = S.misc[0];
+  __syncthreads();                                                        // B1
+  SDIM_TADD(P, 0, tA);
+  SDIM_TMARK(tB);
+  const int piv = M[0];
+  if (tid == 0) { Mn[0] = INT_MAX; Mn[1] = 0; Mn[2] = 0; Mn[3] = 0; }
 
   if (piv != INT_MAX) {
     const uint32_t p = (uint32_t)piv;
-    // ---- gather the pivot column
+    // ---- B: gather the pivot column, touched rows, <x_p, z_p>, pivot phases
     for (uint32_t r = tid; r < n; r += NT) {
-      if constexpr (D == 2) {
-        S.xp[r] = (uint8_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-        S.zp[r] = (uint8_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+      uint32_t xr, zr;
+      if constexpr (BIN) {
+        xr = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
+        zr = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+        bytes += 16;
       } else {
-        S.xp[r] = T.X[(size_t)r * rs + p];
-        S.zp[r] = T.Z[(size_t)r * rs + p];
+        xr = T.X[(size_t)r * rs + p];
+        zr = T.Z[(size_t)r * rs + p];
+        bytes += 2;
       }
-      bytes += (D == 2 ? 16 : 2);
-    }
-    __syncthreads();
-    uint32_t xz = 0;
-    if constexpr (D != 2) {
-      // ---- exponentiate stabilizer p by e = x[q,p]^-1 when x[q,p] != 1 (tableau_prime.py:278-282, 365-380)
-      const uint32_t xv = S.xp[q];
-      uint32_t e = 1;
-      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
-      if (e != 1) {
-        uint32_t part = 0;
-        for (uint32_t r = tid; r < n; r += NT) part += (uint32_t)S.xp[r] * S.zp[r];
-        const uint32_t xz_old = block_sum<NT>(part, S.red) % D;
-        const uint32_t inc = (xz_old * ((e * (e - 1) / 2) % D)) % D;
-        for (uint32_t r = tid; r < n; r += NT) {
-          S.xp[r] = (uint8_t)((S.xp[r] * e) % D);
-          S.zp[r] = (uint8_t)((S.zp[r] * e) % D);
-        }
-        for (uint32_t s = tid; s < Sh; s += NT) {
-          uint8_t* ph = phase_vec(T, s) + p;
-          *ph = (uint8_t)((*ph * e + inc) % D);
-        }
-        __syncthreads();
+      S.xp[r] = (uint8_t)xr;
+      S.zp[r] = (uint8_t)zr;
+      if (xr | zr) {
+        S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
+        if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
       }
-      uint32_t part = 0;
-      for (uint32_t r = tid; r < n; r += NT) part += (uint32_t)S.xp[r] * S.zp[r];
-      xz = block_sum<NT>(part, S.red) % D;
     }
-
-    // ---- rowsum factors f = -x[q, .] (f_p = f_{n+p} = 0) and per-shot pivot phases
-    const uint32_t NWd = (D == 2) ? (G + 63) / 64 : (G + 3) / 4;
-    for (uint32_t w = tid; w < NWd; w += NT) {
-      if constexpr (D == 2) {
-        uint64_t f = row64(T.X, q, rs)[w];
-        if ((p >> 6) == w) f &= ~(1ull << (p & 63));
-        if (((n + p) >> 6) == w) f &= ~(1ull << ((n + p) & 63));
-        ((uint64_t*)S.F)[w] = f;
+    if (tid == 0) {
+      if constexpr (BIN) {
+        ((uint64_t*)S.F)[p >> 6] &= ~(1ull << (p & 63));
+        ((uint64_t*)S.F)[(n + p) >> 6] &= ~(1ull << ((n + p) & 63));
       } else {
-        uint32_t f = negm<D>(row32(T.X, q, rs)[w]);
-        if ((p >> 2) == w) f &= ~(0xFFu << (8 * (p & 3)));
-        if (((n + p) >> 2) == w) f &= ~(0xFFu << (8 * ((n + p) & 3)));
-        ((uint32_t*)S.F)[w] = f;
+        S.F[p] = 0;
+        S.F[n + p] = 0;
       }
-      bytes += (D == 2 ? 8 : 4);
     }
     for (uint32_t s = tid; s < Sh; s += NT) {
-      if constexpr (D == 2) {
-        const uint32_t b0 = (plane(T, s, 0)[p >> 6] >> (p & 63)) & 1;
-        const uint32_t b1 = (plane(T, s, 1)[p >> 6] >> (p & 63)) & 1;
-        S.shot[s] = b0 | (b1 << 1);
+      if constexpr (BIN) {
+        const uint32_t v0 = (uint32_t)((plane(T, s, 0)[p >> 6] >> (p & 63)) & 1);
+        const uint32_t v1 = (uint32_t)((plane(T, s, 1)[p >> 6] >> (p & 63)) & 1);
+        S.shot[s] = v0 | (v1 << 1);
       } else {
         S.shot[s] = phase_vec(T, s)[p];
       }
     }
-    __syncthreads();
+    __syncthreads();                                                      // B2
+    SDIM_TADD(P, 1, tB);
+    SDIM_TMARK(tC);
+    const uint32_t nT = (uint32_t)M[1], nF = (uint32_t)M[2];
+    // exponentiation of stabilizer p by e = x[q,p]^-1 (tableau_prime.py:278-282, 365-380),
+    // folded into every later use of column p
+    uint32_t e = 1, xz = 0, pinc = 0;
+    if constexpr (!BIN) {
+      const uint32_t xv = S.xp[q];
+      #pragma unroll
+      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
+      const uint32_t xz_old = (uint32_t)M[3] % D;
+      xz = ((e * e) % D) * xz_old % D;
+      pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
+    }
 
-    // ---- sweep: rank-1 update of X and Z, column dots <z_i, x_p> from the old Z
-    uint32_t NRG, my_w, my_rg;
-    if (NWd <= NT) { NRG = NT / NWd; my_w = tid % NWd; my_rg = tid / NWd; }
-    else { NRG = 1; my_w = tid; my_rg = 0; }
-    const bool act = my_rg < NRG;
-    constexpr int KW = 4;
-    if constexpr (D == 2) {
-      uint64_t acc[KW];
-      #pragma unroll
-      for (int i = 0; i < KW; i++) acc[i] = 0;
-      if (act) {
-        #pragma unroll
-        for (int kw = 0; kw < KW; kw++) {
-          const uint32_t w = my_w + (uint32_t)kw * NT;
-          if (w < NWd && (kw == 0 || NWd > NT)) {
+    // ---- C: sweep (touched rows) x (words with f != 0)
+    if (nT && nF) {
+      uint32_t NRG, j0, g0, jstep;
+      if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
+      else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
+      if (g0 < NRG) {
+        for (uint32_t j = j0; j < nF; j += jstep) {
+          const uint32_t w = S.FL[j];
+          if constexpr (BIN) {
             const uint64_t f = ((const uint64_t*)S.F)[w];
-            uint64_t a = 0;
-            for (uint32_t r = my_rg; r < n; r += NRG) {
-              const uint32_t sx = S.xp[r], sz = S.zp[r];
-              if (sx | sz) {
-                uint64_t* xr = row64(T.X, r, rs) + w;
-                uint64_t* zr = row64(T.Z, r, rs) + w;
-                const uint64_t z = *zr;
-                if (sx) { a ^= z; *xr ^= f; bytes += 16; }
-                if (sz) { *zr = z ^ f; bytes += 8; }
-                bytes += 8;
-              }
-            }
-            acc[kw] = a;
-          }
-        }
-      }
-      uint64_t* r64 = (uint64_t*)S.red;
-      if (NWd <= NT) {
-        if (act) r64[tid] = acc[0];
-        __syncthreads();
-        for (uint32_t w = tid; w < NWd; w += NT) {
-          uint64_t a = 0;
-          for (uint32_t g = 0; g < NRG; g++) a ^= r64[g * NWd + w];
-          ((uint64_t*)S.C)[w] = a & ((const uint64_t*)S.F)[w];   // po * f * <z_i, x_p> mod 4 -> bit of p1
-        }
-      } else {
-        #pragma unroll
-        for (int kw = 0; kw < KW; kw++) {
-          const uint32_t w = my_w + (uint32_t)kw * NT;
-          if (w < NWd) ((uint64_t*)S.C)[w] = acc[kw] & ((const uint64_t*)S.F)[w];
-        }
-      }
-    } else {
-      uint32_t ae[KW], ao[KW];
-      #pragma unroll
-      for (int i = 0; i < KW; i++) { ae[i] = 0; ao[i] = 0; }
-      if (act) {
-        #pragma unroll
-        for (int kw = 0; kw < KW; kw++) {
-          const uint32_t w = my_w + (uint32_t)kw * NT;
-          if (w < NWd && (kw == 0 || NWd > NT)) {
-            const uint32_t f = ((const uint32_t*)S.F)[w];
-            uint32_t e_ = 0, o_ = 0;
-            uint32_t cnt = 0;
-            for (uint32_t r = my_rg; r < n; r += NRG) {
-              const uint32_t sx = S.xp[r], sz = S.zp[r];
-              if (sx | sz) {
-                uint32_t* xr = row32(T.X, r, rs) + w;
-                uint32_t* zr = row32(T.Z, r, rs) + w;
-                const uint32_t z = *zr;
-                if (sx) {
-                  const uint32_t pz = mult<D>(S.mtab[sx], z);
-                  e_ += pz & 0x00FF00FFu; o_ += (pz >> 8) & 0x00FF00FFu;
-                  *xr = addm<D>(*xr, mult<D>(S.mtab[sx], f));
-                  bytes += 8;
-                  if (++cnt == 4096) {   // keep the 16-bit partial sums from overflowing
-                    e_ = ((e_ & 0xFFFFu) % D) | (((e_ >> 16) % D) << 16);
-                    o_ = ((o_ & 0xFFFFu) % D) | (((o_ >> 16) % D) << 16);
-                    cnt = 0;
-                  }
+            uint64_t dacc = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB];
+              uint64_t xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
+                  zv[b] = row64(T.Z, rr[b], rs)[w];
+                  xv[b] = sxv[b] ? row64(T.X, rr[b], rs)[w] : 0ull;
                 }
-                if (sz) { *zr = addm<D>(z, mult<D>(S.mtab[sz], f)); bytes += 4; }
-                bytes += 4;
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  if (sxv[b]) { dacc ^= zv[b]; row64(T.X, rr[b], rs)[w] = xv[b] ^ f; bytes += 16; }
+                  if (szv[b]) { row64(T.Z, rr[b], rs)[w] = zv[b] ^ f; bytes += 8; }
+                  bytes += 8;
+                }
               }
             }
-            ae[kw] = ((e_ & 0xFFFFu) % D) | (((e_ >> 16) % D) << 16);
-            ao[kw] = ((o_ & 0xFFFFu) % D) | (((o_ >> 16) % D) << 16);
-          }
-        }
-      }
-      // reduce partial dots over row groups; C = f * dot + xz * f (f - 1) / 2 per column
-      if (NWd <= NT) {
-        if (act) { S.red[2 * tid] = ae[0]; S.red[2 * tid + 1] = ao[0]; }
-        __syncthreads();
-        for (uint32_t w = tid; w < NWd; w += NT) {
-          uint32_t e_ = 0, o_ = 0;
-          for (uint32_t g = 0; g < NRG; g++) { e_ += S.red[2 * (g * NWd + w)]; o_ += S.red[2 * (g * NWd + w) + 1]; }
-          const uint32_t f = ((const uint32_t*)S.F)[w];
-          uint32_t c = 0;
-          #pragma unroll
-          for (int kb = 0; kb < 4; kb++) {
-            const uint32_t fb = byte_of(f, kb);
-            const uint32_t dot = ((kb & 1) ? (o_ >> (16 * (kb >> 1))) : (e_ >> (16 * (kb >> 1)))) & 0xFFFFu;
-            const uint32_t cb = (fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D;
-            c |= cb << (8 * kb);
-          }
-          ((uint32_t*)S.C)[w] = c;
-        }
-      } else {
-        #pragma unroll
-        for (int kw = 0; kw < KW; kw++) {
-          const uint32_t w = my_w + (uint32_t)kw * NT;
-          if (w < NWd) {
-            const uint32_t f = ((const uint32_t*)S.F)[w];
-            uint32_t c = 0;
-            #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-              const uint32_t fb = byte_of(f, kb);
-              const uint32_t dot = ((kb & 1) ? (ao[kw] >> (16 * (kb >> 1))) : (ae[kw] >> (16 * (kb >> 1)))) & 0xFFFFu;
-              const uint32_t cb = (fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D;
-              c |= cb << (8 * kb);
+            if (dacc) {
+              atomicXor(&S.DOT[2 * w], (uint32_t)dacc);
+              atomicXor(&S.DOT[2 * w + 1], (uint32_t)(dacc >> 32));
             }
-            ((uint32_t*)S.C)[w] = c;
+          } else {
+            const uint32_t f = ((const uint32_t*)S.F)[w];
+            uint32_t de = 0, dO = 0, cnt = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB], xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = (e * S.xp[rr[b]]) % D;
+                  szv[b] = (e * S.zp[rr[b]]) % D;
+                  zv[b] = row32(T.Z, rr[b], rs)[w];
+                  xv[b] = sxv[b] ? row32(T.X, rr[b], rs)[w] : 0u;
+                }
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  if (sxv[b]) {
+                    const MulTab& t = S.mtab[sxv[b]];
+                    const uint32_t pz = mult<D>(t, zv[b]);
+                    de += pz & 0x00FF00FFu;
+                    dO += (pz >> 8) & 0x00FF00FFu;
+                    row32(T.X, rr[b], rs)[w] = addm<D>(xv[b], mult<D>(t, f));
+                    bytes += 8;
+                  }
+                  if (szv[b]) { row32(T.Z, rr[b], rs)[w] = addm<D>(zv[b], mult<D>(S.mtab[szv[b]], f)); bytes += 4; }
+                  bytes += 4;
+                }
+              }
+              if (++cnt == 1024) { de = fold16<D>(de); dO = fold16<D>(dO); cnt = 0; }
+            }
+            de = fold16<D>(de);
+            dO = fold16<D>(dO);
+            if (de) atomicAdd(&S.DOT[2 * w], de);
+            if (dO) atomicAdd(&S.DOT[2 * w + 1], dO);
           }
         }
       }
     }
-    __syncthreads();
+    __syncthreads();                                                      // B3
+    SDIM_TADD(P, 2, tC);
+    SDIM_TMARK(tD);
 
-    // ---- columns p and n+p: destabilizer p <- stabilizer p, stabilizer p <- Z_q
+    // ---- D: columns p and n+p for every row; per-shot phases over words with f != 0
     for (uint32_t r = tid; r < n; r += NT) {
-      const uint32_t xv = S.xp[r], zv = S.zp[r];
-      if constexpr (D == 2) {
+      if constexpr (BIN) {
+        const uint64_t xv = S.xp[r], zv = S.zp[r];
         uint64_t* xr = row64(T.X, r, rs);
         uint64_t* zr = row64(T.Z, r, rs);
         const uint32_t g = n + p;
         if ((p >> 6) == (g >> 6)) {
           uint64_t x = xr[p >> 6], z = zr[p >> 6];
-          x &= ~(1ull << (p & 63)); x = (x & ~(1ull << (g & 63))) | ((uint64_t)xv << (g & 63));
+          x &= ~(1ull << (p & 63));
+          x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
           z = (z & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          z = (z & ~(1ull << (g & 63))) | ((uint64_t)zv << (g & 63));
+          z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
           xr[p >> 6] = x; zr[p >> 6] = z;
         } else {
           xr[p >> 6] &= ~(1ull << (p & 63));
           zr[p >> 6] = (zr[p >> 6] & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          xr[g >> 6] = (xr[g >> 6] & ~(1ull << (g & 63))) | ((uint64_t)xv << (g & 63));
-          zr[g >> 6] = (zr[g >> 6] & ~(1ull << (g & 63))) | ((uint64_t)zv << (g & 63));
+          xr[g >> 6] = (xr[g >> 6] & ~(1ull << (g & 63))) | (xv << (g & 63));
+          zr[g >> 6] = (zr[g >> 6] & ~(1ull << (g & 63))) | (zv << (g & 63));
         }
         bytes += 32;
       } else {
         T.X[(size_t)r * rs + p] = 0;
         T.Z[(size_t)r * rs + p] = (uint8_t)(r == q);
-        T.X[(size_t)r * rs + n + p] = (uint8_t)xv;
-        T.Z[(size_t)r * rs + n + p] = (uint8_t)zv;
+        T.X[(size_t)r * rs + n + p] = (uint8_t)((e * S.xp[r]) % D);
+        T.Z[(size_t)r * rs + n + p] = (uint8_t)((e * S.zp[r]) % D);
         bytes += 4;
       }
     }
-    // ---- per-shot phases: ph_i += f_i ph_p + C_i
-    for (uint32_t i = tid; i < Sh * NWd; i += NT) {
-      const uint32_t s = i / NWd, w = i - s * NWd;
-      if constexpr (D == 2) {
+    for (uint32_t i = tid; i < Sh * nF; i += NT) {
+      const uint32_t s = i / nF, j = i - s * nF;
+      const uint32_t w = S.FL[j];
+      if constexpr (BIN) {
         const uint32_t k2 = S.shot[s];
         const uint64_t f = ((const uint64_t*)S.F)[w];
+        const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
         uint64_t* p0 = plane(T, s, 0) + w;
         uint64_t* p1 = plane(T, s, 1) + w;
         uint64_t a0 = *p0, a1 = *p1;
         const uint64_t k0m = (k2 & 1) ? f : 0ull, k1m = (k2 & 2) ? f : 0ull;
-        a1 ^= k1m ^ (a0 & k0m); a0 ^= k0m;
-        a1 ^= ((const uint64_t*)S.C)[w];
+        a1 ^= k1m ^ (a0 & k0m);
+        a0 ^= k0m;
+        a1 ^= cdot;                                   // po * f * <z_i, x_p>
         *p0 = a0; *p1 = a1;
         bytes += 32;
       } else {
-        uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
         const uint32_t f = ((const uint32_t*)S.F)[w];
-        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[S.shot[s]], f)), ((const uint32_t*)S.C)[w]);
+        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
+        uint32_t c = 0;
+        #pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+          const uint32_t fb = byte_of(f, kb);
+          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
+          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
+        }
+        const uint32_t php = (e * S.shot[s] + pinc) % D;   // exponentiated pivot phase
+        uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
+        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
         bytes += 8;
       }
     }
-    __syncthreads();
-    // ---- outcome per shot; ph_{n+p} <- ph_p, ph_p <- -m po
+    __syncthreads();                                                      // B4
+    SDIM_TADD(P, 3, tD);
+    SDIM_TMARK(tE);
+    // ---- E: outcome per shot; ph_{n+p} <- ph_p, ph_p <- -m po
     for (uint32_t s = tid; s < Sh; s += NT) {
       const U64x4 o = draw(it.seed, it.stream, src, it.shot0 + s);
       const uint32_t m = outcome_of(o, D);
-      if constexpr (D == 2) {
+      if constexpr (BIN) {
         const uint32_t k2 = S.shot[s];
         const uint32_t g = n + p;
         uint64_t* p0 = plane(T, s, 0);
@@ -703,129 +780,136 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         p1[p >> 6] = (p1[p >> 6] & ~(1ull << (p & 63))) | ((uint64_t)(m & 1) << (p & 63));
       } else {
         uint8_t* ph = phase_vec(T, s);
-        ph[n + p] = (uint8_t)S.shot[s];
+        ph[n + p] = (uint8_t)((e * S.shot[s] + pinc) % D);
         ph[p] = (uint8_t)((D - m) % D);
       }
       S.shot[Sh + s] = m;
       rec[s] = (uint8_t)m;
     }
+    SDIM_TADD(P, 4, tE);
   } else {
-    // ---- deterministic: f_i = dx[q, i] = X[q][n + i]
-    uint32_t part = 0;
-    if constexpr (D == 2) {
-      // f bits aligned to stabilizer columns: F[w] = bits n + 64 w .. of row q
-      const uint32_t nw = (n + 63) / 64;
-      const uint64_t* xr = row64(T.X, q, rs);
-      for (uint32_t w = tid; w < nw; w += NT) {
+    SDIM_TMARK(tDet);
+    // ---- deterministic outcome: prod_i S_i^{f_i}, f_i = dx[q, i] = X[q][n + i]
+    // F holds f aligned to the stabilizer columns (word w = columns w*CPW ...)
+    for (uint32_t w = tid; w < nsw; w += NT) {
+      if constexpr (BIN) {
+        const uint64_t* rq = (const uint64_t*)S.RQ;
         const uint32_t g = n + 64 * w;
-        uint64_t lo = xr[g >> 6] >> (g & 63);
-        uint64_t hi = ((g & 63) && (g >> 6) + 1 < rs / 8) ? xr[(g >> 6) + 1] << (64 - (g & 63)) : 0ull;
+        const uint64_t lo = rq[g >> 6] >> (g & 63);
+        const uint64_t hi = ((g & 63) && (g >> 6) + 1 < rs / 8) ? rq[(g >> 6) + 1] << (64 - (g & 63)) : 0ull;
         uint64_t f = lo | hi;
         const uint32_t lim = n - 64 * w;
         if (lim < 64) f &= (1ull << lim) - 1;
         ((uint64_t*)S.F)[w] = f;
+      } else {
+        uint32_t f = 0;
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const uint32_t c = 4 * w + b;
+          if (c < n) f |= (uint32_t)S.RQ[n + c] << (8 * b);
+        }
+        ((uint32_t*)S.F)[w] = f;
       }
-      __syncthreads();
-      // cross term per row: parity of sum_i (f x)_i * prefix_{j<i} (f z)_j
-      for (uint32_t r = tid; r < n; r += NT) {
-        const uint64_t* xr2 = row64(T.X, r, rs);
-        const uint64_t* zr2 = row64(T.Z, r, rs);
+    }
+    __syncthreads();
+    // cross term sum_r sum_i (f x)_ri * prefix_{j<i} (f z)_rj and the exponent term
+    // sum_i f_i (f_i - 1) / 2 <x_i, z_i>, one thread per row
+    uint32_t part = 0;
+    for (uint32_t r = tid; r < n; r += NT) {
+      if constexpr (BIN) {
+        const uint64_t* xr = row64(T.X, r, rs);
+        const uint64_t* zr = row64(T.Z, r, rs);
         uint64_t carry = 0, par = 0;
-        for (uint32_t w = 0; w < nw; w++) {
+        for (uint32_t w = 0; w < nsw; w++) {
           const uint64_t f = ((const uint64_t*)S.F)[w];
-          const uint64_t a = xr2[w] & f, b = zr2[w] & f;
+          if (!f) continue;
+          const uint64_t a = xr[w] & f, b = zr[w] & f;
           uint64_t y = b;
           y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
           const uint64_t pre = (y << 1) ^ (carry ? ~0ull : 0ull);
           par ^= (uint64_t)(__builtin_popcountll(a & pre) & 1);
           carry ^= (uint64_t)(__builtin_popcountll(b) & 1);
+          bytes += 16;
         }
         part += (uint32_t)par;
-        bytes += 16ull * nw;
-      }
-      const uint32_t K = block_sum<NT>(part, S.red) & 1;
-      for (uint32_t s = tid; s < Sh; s += NT) {
-        uint32_t c0 = 0, c1 = 0;
-        for (uint32_t w = 0; w < nw; w++) {
-          const uint64_t f = ((const uint64_t*)S.F)[w];
-          c0 += __builtin_popcountll(plane(T, s, 0)[w] & f);
-          c1 += __builtin_popcountll(plane(T, s, 1)[w] & f);
+      } else {
+        const uint32_t* xr = row32(T.X, r, rs);
+        const uint32_t* zr = row32(T.Z, r, rs);
+        uint32_t U = 0, acc = 0;
+        for (uint32_t w = 0; w < nsw; w++) {
+          const uint32_t f = ((const uint32_t*)S.F)[w];
+          if (!f) continue;
+          const uint32_t x = xr[w], z = zr[w];
+          #pragma unroll
+          for (int b = 0; b < 4; b++) {
+            const uint32_t fb = byte_of(f, b);
+            const uint32_t xb = byte_of(x, b), zb = byte_of(z, b);
+            acc += fb * xb * U + ((fb * (fb - 1) / 2) % D) * xb * zb;
+            U = (U + fb * zb) % D;
+          }
+          bytes += 8;
         }
-        const uint32_t ap = (c0 + 2 * c1 + 2 * K) & 3;
-        const uint32_t m = ((ap + 1) >> 1) & 1;     // (-ap // 2) mod 2
-        S.shot[Sh + s] = m;
-        rec[s] = (uint8_t)(m | 0x80u);
-        bytes += 16ull * nw;
-      }
-    } else {
-      // f and g = f (f - 1) / 2 per stabilizer column, in F / C
-      for (uint32_t i = tid; i < n; i += NT) {
-        const uint32_t f = T.X[(size_t)q * rs + n + i];
-        S.F[i] = (uint8_t)f;
-        S.C[i] = (uint8_t)((f * (f - 1) / 2) % D);
-        bytes += 1;
-      }
-      __syncthreads();
-      // per row: exclusive scan along the stabilizer columns, one wave per row
-      const uint32_t lane = tid & 63, wid = tid >> 6;
-      const uint32_t cs = (n + 63) / 64;
-      const uint32_t c0 = lane * cs, c1 = (c0 + cs < n) ? c0 + cs : n;
-      for (uint32_t r = wid; r < n; r += NT / 64) {
-        const uint8_t* xr = T.X + (size_t)r * rs;
-        const uint8_t* zr = T.Z + (size_t)r * rs;
-        uint32_t U = 0, V = 0, Lc = 0, SF = 0;
-        for (uint32_t j = c0; j < c1; j++) {
-          const uint32_t f = S.F[j], x = xr[j], z = zr[j];
-          const uint32_t fx = (f * x) % D;
-          Lc = (Lc + fx * U) % D;
-          U = (U + f * z) % D;
-          V = (V + fx) % D;
-          SF = (SF + S.C[j] * ((x * z) % D)) % D;
-        }
-        bytes += 2ull * (c1 > c0 ? c1 - c0 : 0);
-        // exclusive scan of U across lanes (mod D)
-        uint32_t incl = U;
-        #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t v = __shfl_up(incl, off, 64);
-          if (lane >= (uint32_t)off) incl = (incl + v) % D;
-        }
-        const uint32_t excl = (incl + D - U) % D;
-        part += (Lc + V * excl + SF) % D;
-      }
-      const uint32_t K = block_sum<NT>(part % D, S.red) % D;
-      for (uint32_t s = tid; s < Sh; s += NT) {
-        const uint8_t* ph = phase_vec(T, s);
-        uint32_t acc = 0;
-        for (uint32_t i = 0; i < n; i++) acc += (uint32_t)S.F[i] * ph[i];
-        const uint32_t ap = (acc + K) % D;
-        const uint32_t m = (D - ap) % D;
-        S.shot[Sh + s] = m;
-        rec[s] = (uint8_t)(m | 0x80u);
-        bytes += n;
+        part += acc % D;
       }
     }
+    const uint32_t K = BIN ? (block_sum<NT>(part, S.red) & 1) : (block_sum<NT>(part % D, S.red) % D);
+    for (uint32_t s = tid; s < Sh; s += NT) {
+      uint32_t m;
+      if constexpr (BIN) {
+        uint32_t c0 = 0, c1 = 0;
+        for (uint32_t w = 0; w < nsw; w++) {
+          const uint64_t f = ((const uint64_t*)S.F)[w];
+          if (!f) continue;
+          c0 += __builtin_popcountll(plane(T, s, 0)[w] & f);
+          c1 += __builtin_popcountll(plane(T, s, 1)[w] & f);
+          bytes += 16;
+        }
+        const uint32_t ap = (c0 + 2 * c1 + 2 * K) & 3;
+        m = ((ap + 1) >> 1) & 1;                       // (-ap // 2) mod 2
+      } else {
+        const uint32_t* ph = (const uint32_t*)phase_vec(T, s);
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < nsw; w++) {
+          const uint32_t f = ((const uint32_t*)S.F)[w];
+          if (!f) continue;
+          const uint32_t v = ph[w];
+          #pragma unroll
+          for (int b = 0; b < 4; b++) acc += byte_of(f, b) * byte_of(v, b);
+          bytes += 4;
+        }
+        m = (D - (acc + K) % D) % D;
+      }
+      S.shot[Sh + s] = m;
+      rec[s] = (uint8_t)(m | 0x80u);
+    }
+    SDIM_TADD(P, 5, tDet);
   }
-  __syncthreads();
 
   if (reset) {
+    SDIM_TMARK(tR);
+    __syncthreads();
     // X^k with k = (-m) mod d: phase -= k po z[q, .]   (program.py:334-339)
-    const uint32_t NWd = (D == 2) ? (G + 63) / 64 : (G + 3) / 4;
     for (uint32_t i = tid; i < Sh * NWd; i += NT) {
       const uint32_t s = i / NWd, w = i - s * NWd;
       const uint32_t m = S.shot[Sh + s];
-      if constexpr (D == 2) {
-        if (m & 1) { plane(T, s, 1)[w] ^= row64(T.Z, q, rs)[w]; bytes += 24; }
+      if constexpr (BIN) {
+        if (m & 1) {
+          const uint64_t z = row64(T.Z, q, rs)[w];
+          if (z) { plane(T, s, 1)[w] ^= z; bytes += 24; }
+        }
       } else {
-        const uint32_t k = (D - m) % D;
-        if (k) {
-          uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
-          *ph = subm<D>(*ph, mult<D>(S.mtab[k], row32(T.Z, q, rs)[w]));
-          bytes += 12;
+        const uint32_t kk = (D - m) % D;
+        if (kk) {
+          const uint32_t z = row32(T.Z, q, rs)[w];
+          if (z) {
+            uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
+            *ph = subm<D>(*ph, mult<D>(S.mtab[kk], z));
+            bytes += 12;
+          }
         }
       }
     }
-    __syncthreads();
+    SDIM_TADD(P, 6, tR);
   }
 }
 
@@ -838,12 +922,17 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   uint8_t* gtab = P.arena + it.tab_off;
   uint8_t* tab = res ? lds : gtab;
   uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : lds;
-  const Scr S = carve<NT>(sbase, n, rs, Sh);
+  Scr S = carve<NT>(sbase, n, rs, Sh);
+  S.lds_end = lds + P.lds_bytes;
+  S.tiling = !res;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  unsigned long long bytes = 0;
+  T.wofs = 0; T.rs_full = rs;
+  unsigned long long bytes = 0, lbytes = 0;
+  SDIM_TMARK(tItem);
 
   if (threadIdx.x < 16) S.mtab[threadIdx.x] = make_tab<D>(threadIdx.x < D ? threadIdx.x : 0);
+  if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
   if (it.flags & ITEM_INIT_GIVEN) {
     if (res) copy16<NT>(lds, gtab, T_bytes);
     __syncthreads();
@@ -851,34 +940,42 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
     init_identity<D, NT>(T, T_bytes);
   }
   if (threadIdx.x == 0) bytes += T_bytes;
+  SDIM_TADD(P, 9, tItem);
 
   const DevOp* ops = P.ops + it.ops_off;
-  uint32_t k = 0;
+  uint32_t k = 0, mk = 0;
   while (k < it.n_ops) {
     const DevOp o = ops[k];
     const uint32_t kind = op_kind(o);
     if (kind == OP_SEG) {
-      k = run_segment<D, NT>(ops, k, T, S, it, P, bytes);
+      k = run_segment<D, NT>(ops, k, T, S, it, P, bytes, lbytes);
     } else if (kind == OP_MEAS || kind == OP_RESET) {
-      measure<D, NT>(T, o.q, o.t, o.src, kind == OP_RESET, S, it, P, bytes);
+      measure<D, NT>(T, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, bytes);
       k++;
     } else {
       break;   // OP_END
     }
   }
   __syncthreads();
+  SDIM_TMARK(tOut);
   if (res) {
     copy16<NT>(gtab, lds, T_bytes);
     if (threadIdx.x == 0) bytes += T_bytes;
-    __syncthreads();
   }
+  SDIM_TADD(P, 10, tOut);
+  SDIM_TADD(P, 11, tItem);
   const unsigned long long tot = block_sum64<NT>(bytes, S.red);
-  if (threadIdx.x == 0) atomicAdd(&P.counters[res ? 1 : 0], tot);
+  const unsigned long long ltot = block_sum64<NT>(lbytes, S.red);
+  if (threadIdx.x == 0) {
+    atomicAdd(&P.counters[res ? 1 : 0], tot);
+    if (ltot) atomicAdd(&P.counters[1], ltot);
+  }
 }
 
 // ------------------------------------------------------------------ persistent batch kernel
+// Each workgroup pulls items from P.queue until the queue is drained.
 template <int NT>
-__global__ __launch_bounds__(NT) void batch_kernel(KParams P) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(KParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t s_item;
   for (;;) {
@@ -909,7 +1006,7 @@ __global__ __launch_bounds__(256) void plain_init_kernel(KParams P) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)it.n * it.rs; T.Ph = tab + 2 * (size_t)it.n * it.rs;
-  T.n = it.n; T.rs = it.rs; T.S = it.S;
+  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs;
   init_identity<D, 256>(T, tableau_bytes(it.n, D, it.S));
 }
 
@@ -927,8 +1024,11 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.wofs = 0; T.rs_full = rs;
   Scr S;
   S.mtab = mt;
+  S.tiling = false;
+  S.lds_end = nullptr;
   const uint32_t kind = op_kind(o);
   if (kind == OP_NOISE) {
     // per shot draw and apply: phase += po (b x - a z)
@@ -949,21 +1049,26 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
     }
     return;
   }
+  const bool two = two_rows(o);
   if constexpr (D == 2) {
     uint64_t d0 = 0, d1 = 0;
-    gate_bin(kind, o.q, o.t, op_a(o), op_b(o), T, w, d0, d1, S, 0);
+    const uint64_t xq = row64(T.X, o.q, rs)[w], zq = row64(T.Z, o.q, rs)[w];
+    const uint64_t xt = two ? row64(T.X, o.t, rs)[w] : 0ull, zt = two ? row64(T.Z, o.t, rs)[w] : 0ull;
+    apply_bin(o, xq, zq, xt, zt, T, w, d0, d1, S);
     if (d0 | d1) {
       for (uint32_t s = 0; s < Sh; s++) {
         uint64_t* p0 = plane(T, s, 0) + w;
         uint64_t* p1 = plane(T, s, 1) + w;
-        uint64_t a0 = *p0, a1 = *p1;
+        const uint64_t a0 = *p0, a1 = *p1;
         *p1 = a1 ^ d1 ^ (a0 & d0);
         *p0 = a0 ^ d0;
       }
     }
   } else {
     uint32_t dl = 0;
-    gate_odd<D>(kind, o.q, o.t, op_a(o), op_b(o), T, w, dl, S, 0);
+    const uint32_t xq = row32(T.X, o.q, rs)[w], zq = row32(T.Z, o.q, rs)[w];
+    const uint32_t xt = two ? row32(T.X, o.t, rs)[w] : 0u, zt = two ? row32(T.Z, o.t, rs)[w] : 0u;
+    apply_odd<D>(o, xq, zq, xt, zt, T, w, dl, S);
     if (dl) {
       for (uint32_t s = 0; s < Sh; s++) {
         uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
@@ -981,11 +1086,13 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.wofs = 0; T.rs_full = rs;
   const Scr S = carve<NT>(lds, n, rs, Sh);
   if (threadIdx.x < 16) S.mtab[threadIdx.x] = make_tab<D>(threadIdx.x < D ? threadIdx.x : 0);
+  if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
   __syncthreads();
   unsigned long long bytes = 0;
-  measure<D, NT>(T, o.q, o.t, o.src, op_kind(o) == OP_RESET, S, it, P, bytes);
+  measure<D, NT>(T, o.q, o.t, o.src, op_kind(o) == OP_RESET, 0, S, it, P, bytes);
 }
 
 }  // namespace sdim
@@ -994,12 +1101,14 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
 namespace sdim {
 
 __global__ void reset_kernel(uint32_t* queue, unsigned long long* counters) {
-  if (threadIdx.x < 4) queue[threadIdx.x] = 0;
-  if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+  if (threadIdx.x < 8) queue[threadIdx.x] = 0;
+  if (threadIdx.x < 2 + SDIM_NUM_TIMERS) counters[threadIdx.x] = 0;
 }
 
 int prepare_kernels() {
   const int lds = 163584;
+  if (hipFuncSetAttribute((const void*)batch_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)batch_kernel<128>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
@@ -1018,6 +1127,8 @@ int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st) 
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st) {
   const size_t lds = P.lds_bytes;
   switch (nt) {
+    case 64: hipLaunchKernelGGL(batch_kernel<64>, dim3(grid), dim3(64), lds, st, P); break;
+    case 128: hipLaunchKernelGGL(batch_kernel<128>, dim3(grid), dim3(128), lds, st, P); break;
     case 256: hipLaunchKernelGGL(batch_kernel<256>, dim3(grid), dim3(256), lds, st, P); break;
     case 512: hipLaunchKernelGGL(batch_kernel<512>, dim3(grid), dim3(512), lds, st, P); break;
     case 1024: hipLaunchKernelGGL(batch_kernel<1024>, dim3(grid), dim3(1024), lds, st, P); break;

@@ -78,6 +78,34 @@ struct Alloc {
   size_t bytes = 0;
 };
 
+// Launch classes of the persistent kernel.  Resident classes keep the whole
+// tableau in LDS; the others keep it in HBM and use LDS for scratch only.
+// An item goes to the first class whose LDS budget holds what it needs.
+struct LaunchClass {
+  bool resident;
+  int nt;          // threads per workgroup
+  uint32_t lds;    // dynamic LDS bytes per workgroup
+  int per_cu;      // workgroups resident per CU
+};
+// default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu")
+std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
+  auto budget = [](int per_cu) { return (uint32_t)((163840 / per_cu - 256) & ~15); };
+  return {
+      {true, 128, budget(8), 8},
+      {true, 256, budget(4), 4},
+      {true, 512, budget(2), 2},
+      {true, 1024, 163584, 1},
+      {false, hbm_nt, budget(hbm_per_cu), hbm_per_cu},
+      {false, 256, budget(4), 4},
+      {false, 1024, 163584, 1},
+  };
+}
+
+struct Group {
+  int cls = 0;
+  uint32_t first = 0, count = 0;
+};
+
 struct Ticket {
   uint64_t id = 0;
   int mode = 0;
@@ -91,14 +119,15 @@ struct Ticket {
   std::vector<uint8_t> h_meta;                      // staging kept alive until the ticket finishes
   std::vector<std::vector<uint8_t>> h_inits;
   unsigned long long* d_counters = nullptr;
-  unsigned long long h_counters[2] = {0, 0};
+  unsigned long long h_counters[2 + SDIM_NUM_TIMERS] = {0};
   bool finished = false;
   int64_t n_lds_items = 0;
   // launch plan (re-enqueued by sdim_replay)
   KParams P{};
-  uint32_t n_hbm_items = 0;
   uint32_t* d_queue = nullptr;
-  int nt = 1024, grid = 256;
+  std::vector<Group> groups;
+  std::vector<LaunchClass> classes;
+  int n_cu = 256, grid_override = 0, threads = 0;
   hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
 };
 
@@ -113,14 +142,18 @@ struct sdim_ctx {
   std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
   std::multimap<size_t, void*> free_blocks;   // caching allocator
   // options
-  int64_t threads = 1024;
+  int64_t threads = 0;       // 0: per-class workgroup size
   int64_t lds_bytes = 163584;
   int64_t shots_per_item = 0;
   int64_t grid = 0;
   int64_t allow_lds = 1;
+  int64_t hbm_threads = 512;
+  int64_t hbm_per_cu = 2;
 };
 
 namespace {
+
+uint32_t nt_of(const sdim_ctx* c, const LaunchClass& L) { return c->threads ? (uint32_t)c->threads : (uint32_t)L.nt; }
 
 size_t round_block(size_t b) {
   size_t r = 1 << 20;
@@ -389,18 +422,20 @@ int enqueue(Ticket* T, int which) {
   KParams& P = T->P;
   if (T->mode == SDIM_MODE_BATCHED) {
     if (launch_reset(T->d_queue, P.counters, T->stream)) return fail(SDIM_EHIP, "reset launch failed");
-    if (T->n_hbm_items && which != 2) {
+    for (const Group& g : T->groups) {
+      const LaunchClass& L = T->classes[g.cls];
+      if (which == 1 && L.resident) continue;
+      if (which == 2 && !L.resident) continue;
       KParams Q = P;
-      Q.n_items = T->n_hbm_items;
-      Q.queue = T->d_queue;
-      if (launch_batch(Q, T->nt, T->grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
-    }
-    if (T->n_lds_items && which != 1) {
-      KParams Q = P;
-      Q.order = P.order + T->n_hbm_items;
-      Q.n_items = (uint32_t)T->n_lds_items;
-      Q.queue = T->d_queue + 1;
-      if (launch_batch(Q, T->nt, T->grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
+      Q.order = P.order + g.first;
+      Q.n_items = g.count;
+      Q.queue = T->d_queue + g.cls;
+      Q.lds_bytes = std::min<uint32_t>(L.lds, P.lds_bytes);
+      const int nt = T->threads ? T->threads : L.nt;
+      const int per_cu = std::max(1, std::min(L.per_cu, 1024 / nt));
+      const int grid = T->grid_override ? T->grid_override
+                                        : (int)std::min<uint64_t>((uint64_t)T->n_cu * per_cu, g.count);
+      if (launch_batch(Q, nt, grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
     }
     return 0;
   }
@@ -475,7 +510,7 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   std::lock_guard<std::mutex> lk(c->mu);
   const std::string k(key);
   if (k == "threads") {
-    if (v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "threads must be 256, 512 or 1024");
+    if (v != 0 && v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "threads must be 0, 256, 512 or 1024");
     c->threads = v;
   } else if (k == "lds_bytes") {
     if (v < 16384 || v > 163584) return fail(SDIM_EINVAL, "lds_bytes out of range");
@@ -488,6 +523,12 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
     c->grid = v;
   } else if (k == "allow_lds") {
     c->allow_lds = v ? 1 : 0;
+  } else if (k == "hbm_threads") {
+    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "hbm_threads must be 64..1024");
+    c->hbm_threads = v;
+  } else if (k == "hbm_per_cu") {
+    if (v < 1 || v > 16) return fail(SDIM_EINVAL, "hbm_per_cu must be 1..16");
+    c->hbm_per_cu = v;
   } else {
     return fail(SDIM_EINVAL, "unknown option " + k);
   }
@@ -503,6 +544,8 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "shots_per_item") *v = c->shots_per_item;
   else if (k == "grid") *v = c->grid ? c->grid : c->n_cu;
   else if (k == "allow_lds") *v = c->allow_lds;
+  else if (k == "hbm_threads") *v = c->hbm_threads;
+  else if (k == "hbm_per_cu") *v = c->hbm_per_cu;
   else if (k == "num_cu") *v = c->n_cu;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
@@ -517,13 +560,21 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   T->mode = mode;
   T->stream = c->stream;
   T->jobs.resize((size_t)n_jobs);
-  const uint32_t NT = (uint32_t)c->threads;
-  const uint32_t budget = (uint32_t)c->lds_bytes;
+  const uint32_t cap = (uint32_t)c->lds_bytes;
+  T->classes = make_classes((int)c->hbm_threads, (int)c->hbm_per_cu);
+  const std::vector<LaunchClass>& kClasses = T->classes;
+  const int kNumClasses = (int)kClasses.size();
+  uint64_t total_shots = 0;
+  for (int32_t ji = 0; ji < n_jobs; ji++) total_shots += (uint64_t)std::max<int64_t>(jobs[ji].shots, 0);
+  // automatic shots per item: enough items to keep every workgroup slot busy twice over
+  const uint64_t slots = (uint64_t)c->n_cu * 4;
+  const uint32_t auto_S = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, total_shots / (2 * slots)));
 
   std::vector<uint64_t> keep;
   std::vector<DevOp> ops;
   std::vector<ItemDesc>& items = T->items;
   std::vector<uint64_t> cost;
+  std::vector<int> cls_of;
   std::vector<std::pair<uint32_t, std::vector<uint8_t>>> inits;   // item -> initial tableau bytes
 
   uint64_t rec_total = 0;
@@ -534,6 +585,7 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     if (j.d < 2) return fail(SDIM_EINVAL, "Dimension must be greater than 1");
     if (!supported_dim(j.d)) return fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})");
     if (j.shots < 1) return fail(SDIM_EINVAL, "shots must be >= 1");
+    if (j.n > 32768) return fail(SDIM_EINVAL, "at most 32768 qudits are supported");
     if (j.n_ops < 0 || (j.n_ops && !j.ir)) return fail(SDIM_EINVAL, "bad op list");
     jp.n = j.n;
     jp.d = j.d;
@@ -542,45 +594,46 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     const size_t keep0 = keep.size();
     int rc = expand(j, jp, keep);
     if (rc) return rc;
-    for (size_t k = keep0; k < keep.size(); k++) (void)k;
     // keep indices in prims are global; make them circuit-relative
     for (auto& p : jp.prims) if (p.kind == OP_NOISE) p.aux -= (uint32_t)keep0;
 
     const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = row_stride(n, d);
     const bool has_noise = keep.size() > keep0;
-    // shots per item
-    uint32_t S = 1;
-    bool resident = false;
-    const uint32_t want = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
-    if (mode == SDIM_MODE_BATCHED && c->allow_lds) {
-      // largest S <= want with tableau + scratch (+ one noise op) inside the LDS budget
-      uint32_t lo = 0, hi = want;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        const uint64_t need = ((tableau_bytes(n, d, mid) + 15) & ~15ull) + scratch_bytes(n, rs, mid, NT) +
-                              (has_noise ? noise_bytes(1, rs, mid) : 0);
-        if (need <= budget) lo = mid; else hi = mid - 1;
+    uint32_t S = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : auto_S);
+    int cls = -1;
+    if (mode == SDIM_MODE_BATCHED) {
+      auto need = [&](int k, uint32_t s) -> uint64_t {
+        const LaunchClass& L = kClasses[k];
+        return (L.resident ? ((tableau_bytes(n, d, s) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, s, nt_of(c, L)) +
+               (has_noise ? noise_bytes(1, rs, s) : 0);
+      };
+      for (int k = 0; k < kNumClasses && cls < 0; k++) {
+        const LaunchClass& L = kClasses[k];
+        if (L.resident && !c->allow_lds) continue;
+        if (need(k, S) <= std::min<uint32_t>(L.lds, cap)) cls = k;
       }
-      if (lo >= 1 && (c->shots_per_item ? lo >= want : true)) { S = lo; resident = true; }
-    }
-    if (!resident) {
-      S = want;
-      if (mode == SDIM_MODE_BATCHED) {
-        // scratch (+ one noise op) of an HBM-resident item must fit in LDS
-        while (S > 1 && scratch_bytes(n, rs, S, NT) + (has_noise ? noise_bytes(1, rs, S) : 0) > budget) S /= 2;
-        if (scratch_bytes(n, rs, S, NT) + (has_noise ? noise_bytes(1, rs, S) : 0) > budget)
+      if (cls < 0) {
+        // the last class keeps the tableau in HBM with the largest scratch: shrink S until it fits
+        const int k = kNumClasses - 1;
+        while (S > 1 && need(k, S) > std::min<uint32_t>(kClasses[k].lds, cap)) S /= 2;
+        if (need(k, S) > std::min<uint32_t>(kClasses[k].lds, cap))
           return fail(SDIM_ENOMEM, "circuit too large for the LDS scratch budget");
-      } else {
-        if (scratch_bytes(n, rs, S, 256) > 163584) return fail(SDIM_ENOMEM, "circuit too large for the plain path");
+        cls = k;
       }
+    } else {
+      if (scratch_bytes(n, rs, S, 256) > 163584) return fail(SDIM_ENOMEM, "circuit too large for the plain path");
     }
+    const bool resident = cls >= 0 && kClasses[cls].resident;
     jp.S_item = S;
     jp.resident = resident;
     // noise ops per segment that fit next to the tableau / scratch
-    uint32_t nn_max = 0;
     if (mode == SDIM_MODE_BATCHED) {
-      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, S, NT);
-      const uint64_t room = budget > base ? budget - base : 0;
+      const LaunchClass& L = kClasses[cls];
+      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, S, nt_of(c, L));
+      const uint64_t budget = std::min<uint32_t>(L.lds, cap);
+      // HBM-resident items keep half of the remaining LDS for the column tiles of gate segments
+      const uint64_t room = budget > base ? (resident ? budget - base : (budget - base) / 2) : 0;
+      uint32_t nn_max = 0;
       while (nn_max < 65535 && noise_bytes(nn_max + 1, rs, S) <= room) nn_max++;
       if (has_noise && nn_max == 0) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
       jp.ops_off = (uint32_t)ops.size();
@@ -589,7 +642,6 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     }
     // items
     jp.first_item = (uint32_t)items.size();
-    const uint64_t tb = tableau_bytes(n, d, S);
     std::vector<uint8_t> init_buf;
     const bool given = j.init[0] && j.init[1] && j.init[2] && j.init[3] && j.init[4] && j.init[5];
     if (given) pack_tableau(jp, S, j, init_buf);
@@ -612,10 +664,9 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
       if (given) inits.emplace_back((uint32_t)items.size(), init_buf);
       items.push_back(it);
-      // LPT cost: measurement sweeps dominate (rows * row bytes * 3 per measurement) + phase passes
-      const uint64_t sweep = (uint64_t)jp.n_meas * 3ull * n * rs + (uint64_t)it.S * rs * (jp.n_meas + 4) + jp.prims.size() * 8ull * rs;
-      cost.push_back(resident ? sweep / 6 : sweep);
-      (void)tb;
+      cls_of.push_back(cls);
+      // LPT cost estimate: measurement sweeps (rows x row bytes) + per-shot phase passes + gates
+      cost.push_back((uint64_t)jp.n_meas * n * rs + (uint64_t)it.S * rs * (jp.n_meas + 4) + jp.prims.size() * 4ull * rs);
     }
     jp.n_items = (uint32_t)items.size() - jp.first_item;
   }
@@ -627,21 +678,34 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     off = align256(off + tableau_bytes(it.n, it.d, it.S));
   }
   const uint64_t arena_bytes = std::max<uint64_t>(off, 256);
-  // metadata: items, order, ops, keep, queue, counters
+  // metadata: items, order, ops, keep, queue, counters.  Order: by launch class, longest first inside a class.
   std::vector<uint32_t> order(items.size());
   for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-    const bool ra = items[a].flags & ITEM_LDS, rb = items[b].flags & ITEM_LDS;
-    if (ra != rb) return !ra;          // HBM-resident items first
-    return cost[a] > cost[b];          // longest first inside a class
+    if (cls_of[a] != cls_of[b]) return cls_of[a] < cls_of[b];
+    return cost[a] > cost[b];
   });
+  T->groups.clear();
+  if (mode == SDIM_MODE_BATCHED) {
+    for (uint32_t i = 0; i < order.size();) {
+      const int k = cls_of[order[i]];
+      uint32_t e = i;
+      while (e < order.size() && cls_of[order[e]] == k) e++;
+      Group g;
+      g.cls = k;
+      g.first = i;
+      g.count = e - i;
+      T->groups.push_back(g);
+      i = e;
+    }
+  }
   const uint64_t o_items = 0;
   const uint64_t o_order = align256(o_items + items.size() * sizeof(ItemDesc));
   const uint64_t o_ops = align256(o_order + order.size() * 4);
   const uint64_t o_keep = align256(o_ops + ops.size() * sizeof(DevOp));
   const uint64_t o_queue = align256(o_keep + keep.size() * 8);
-  const uint64_t o_cnt = align256(o_queue + 16);
-  const uint64_t meta_bytes = align256(o_cnt + 16);
+  const uint64_t o_cnt = align256(o_queue + 32);
+  const uint64_t meta_bytes = align256(o_cnt + 8 * (2 + SDIM_NUM_TIMERS));
   int rc = ctx_alloc(c, arena_bytes, T->arena);
   if (rc) return rc;
   rc = ctx_alloc(c, meta_bytes, T->meta);
@@ -674,12 +738,12 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   P.keep53 = (const uint64_t*)(meta + o_keep);
   P.arena = arena;
   P.counters = T->d_counters;
-  P.lds_bytes = budget;
+  P.lds_bytes = cap;
   T->d_queue = P.queue;
   for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
-  T->n_hbm_items = (uint32_t)items.size() - (uint32_t)T->n_lds_items;
-  T->nt = (int)NT;
-  T->grid = (int)(c->grid ? c->grid : c->n_cu);
+  T->n_cu = c->n_cu;
+  T->grid_override = (int)c->grid;
+  T->threads = (int)c->threads;
 
   HIP_OK(hipEventRecord(T->ev0, T->stream));
   rc = enqueue(T.get(), 0);
@@ -688,7 +752,7 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   // records back to the host (async; valid after wait)
   T->h_rec.resize(T->rec_bytes);
   if (T->rec_bytes) HIP_OK(hipMemcpyAsync(T->h_rec.data(), arena, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
-  HIP_OK(hipMemcpyAsync(T->h_counters, T->d_counters, 16, hipMemcpyDeviceToHost, T->stream));
+  HIP_OK(hipMemcpyAsync(T->h_counters, T->d_counters, sizeof(T->h_counters), hipMemcpyDeviceToHost, T->stream));
 
   T->id = c->next_ticket++;
   *ticket = T->id;
@@ -815,6 +879,18 @@ int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t*
   return 0;
 }
 
+int sdim_ticket_counters(sdim_ctx* c, uint64_t ticket, uint64_t* out, int32_t count) {
+  if (!c || !out || count < 0) return fail(SDIM_EINVAL, "bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  const int32_t m = std::min<int32_t>(count, 2 + SDIM_NUM_TIMERS);
+  for (int32_t i = 0; i < m; i++) out[i] = t->h_counters[i];
+  return m;
+}
+
 int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms) {
   if (!c || iters < 1 || which < 0 || which > 2) return fail(SDIM_EINVAL, "bad replay arguments");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -846,7 +922,7 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
     }
   }
   HIP_OK(hipEventRecord(t->ev1, t->stream));
-  HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, 16, hipMemcpyDeviceToHost, t->stream));
+  HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, sizeof(t->h_counters), hipMemcpyDeviceToHost, t->stream));
   if (t->rec_bytes)
     HIP_OK(hipMemcpyAsync(t->h_rec.data(), t->arena.p, t->rec_bytes, hipMemcpyDeviceToHost, t->stream));
   HIP_OK(hipStreamSynchronize(t->stream));

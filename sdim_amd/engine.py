@@ -116,6 +116,14 @@ class Ticket:
         return {"kernel_ms": ms.value, "bytes_hbm": bh.value, "bytes_lds": bl.value,
                 "items": ni.value, "lds_items": nl.value}
 
+    def counters(self):
+        """Raw device counters: bytes (HBM-resident, LDS-resident) and phase timer ticks."""
+        buf = (ctypes.c_uint64 * 14)()
+        m = self.engine.L.sdim_ticket_counters(self.engine.ctx, self.id, buf, 14)
+        if m < 0:
+            _lib.check(m)
+        return [int(buf[i]) for i in range(m)]
+
     def replay(self, iters=1, which="all"):
         """Re-run the device work `iters` times; returns device milliseconds for all iterations."""
         w = {"all": 0, "hbm": 1, "lds": 2}[which]
