@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--shots-per-item", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="requests/s (0: 60%% of a measured burst rate)")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB) to calibrate "
                          "rocprof FETCH_SIZE / WRITE_SIZE")
@@ -96,6 +100,39 @@ def cpu_baseline(jobs, budget_s):
                       f"interpreted per-gate loops) on the first {done_jobs} circuit(s) of the workload "
                       f"(n={ns}), 1 shot each, {done_ops} gate-shots in {t_total:.1f} s",
             "host_cores_available": len(os.sched_getaffinity(0))}
+
+
+def serving(eng, n_requests, rate, budget_gb, seed=5):
+    """Serving mix: Poisson arrivals of random circuits, n log-uniform in [8, 2048], d in {2,3,5,7,11},
+    depth 8n, shots log-uniform in [1, 4096]; admission against an HBM budget; continuous batching."""
+    from sdim_amd import workloads as W
+    from sdim_amd.serve import Server, poisson_run
+    rng = np.random.default_rng(seed)
+    reqs = [W.serving_request(rng, k, seed=seed) for k in range(n_requests)]
+    gate_shots = W.count_gate_shots(reqs)
+    if rate <= 0:
+        # burst calibration: everything submitted at once, completion rate measured
+        srv = Server(eng, hbm_budget_bytes=int(budget_gb * (1 << 30)))
+        burst = reqs[:min(60, n_requests)]
+        t0 = time.perf_counter()
+        ids = [srv.submit(r) for r in burst]
+        for rid in ids:
+            srv.wait(rid)
+            srv.result(rid)
+        rate = 0.6 * len(burst) / (time.perf_counter() - t0)
+        srv.close()
+    srv = Server(eng, hbm_budget_bytes=int(budget_gb * (1 << 30)))
+    ids = poisson_run(srv, lambda k: reqs[k], rate, n_requests, seed=seed)
+    for rid in ids:
+        srv.wait(rid)
+        srv.result(rid)
+    st = srv.stats()
+    srv.close()
+    return {"requests": n_requests, "arrival_rate_per_s": rate, "p50_latency_s": st["p50_s"],
+            "p99_latency_s": st["p99_s"], "gate_shots": gate_shots,
+            "aggregate_gate_shots_per_s": st["gate_shots"] / st["span_s"] if st["span_s"] else None,
+            "span_s": st["span_s"], "hbm_budget_gb": budget_gb,
+            "workload": "Poisson arrivals; n log-U[8,2048], d in {2,3,5,7,11}, depth 8n, shots log-U[1,4096]"}
 
 
 def main():
@@ -188,6 +225,9 @@ def main():
                         "batched_same_subset": gs / t_b}
         out["speedup_vs_plain"] = value / (gs / t_plain)
         out["speedup_vs_plain_same_subset"] = (gs / t_b) / (gs / t_plain)
+
+    if not args.no_serve:
+        out["serving"] = serving(eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_seconds)

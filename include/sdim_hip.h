@@ -143,6 +143,38 @@ int sdim_ticket_counters(sdim_ctx* ctx, uint64_t ticket, uint64_t* out, int32_t 
  */
 int sdim_replay(sdim_ctx* ctx, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms);
 
+/* ------------------------------------------------------------------ serving
+ * Continuous batching of independent requests (one circuit each, starting from
+ * |0...0>).  No reference counterpart: the reference simulates one Program at a time.
+ * A dispatcher thread admits pending requests in arrival order while their estimated
+ * device footprint (tableaux + records + gate lists) fits `hbm_budget_bytes`, launches
+ * each admitted group as one submission on a free stream (at most `max_inflight`
+ * submissions in flight, each on its own stream, so new requests overlap running
+ * ones), and on completion copies the records to host memory and frees the device
+ * memory.  Each request keeps its own Philox key (seed, stream): its results are the
+ * same alone or in any mix.
+ */
+typedef struct sdim_server sdim_server;
+
+int sdim_server_create(sdim_ctx* ctx, uint64_t hbm_budget_bytes, int32_t max_inflight, int32_t max_batch,
+                       sdim_server** out);
+/* Copies the request; its arrival time (for latency) is the time of this call. */
+int sdim_server_submit(sdim_server* server, const sdim_job* job, uint64_t* request_id);
+/* state: 0 pending, 1 on the device, 2 done, 3 failed */
+int sdim_server_poll(sdim_server* server, uint64_t request_id, int32_t* state);
+int sdim_server_wait(sdim_server* server, uint64_t request_id, double timeout_s);
+/* Records as in sdim_fetch_records (event-major, then shot); out = NULL queries n_records.
+ * A successful fetch with out != NULL releases the request. */
+int sdim_server_result(sdim_server* server, uint64_t request_id, void* out, int64_t capacity,
+                       int64_t* n_records, double* latency_s);
+/* Completed / pending / in-flight counts, gate-shots of completed requests, time from the
+ * first arrival to the last completion, and the p50 / p99 request latency (arrival to
+ * records on the host). */
+int sdim_server_stats(sdim_server* server, int64_t* completed, int64_t* pending, int64_t* inflight,
+                      uint64_t* gate_shots, double* span_s, double* p50_s, double* p99_s);
+/* Drains every pending and in-flight request, then stops the dispatcher. */
+int sdim_server_destroy(sdim_server* server);
+
 #ifdef __cplusplus
 }
 #endif

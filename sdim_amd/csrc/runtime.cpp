@@ -136,7 +136,8 @@ struct Ticket {
 struct sdim_ctx {
   int device = 0;
   int n_cu = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;               // streams[0]
+  std::vector<hipStream_t> streams;           // all default priority
   std::mutex mu;
   uint64_t next_ticket = 1;
   std::map<uint64_t, std::unique_ptr<Ticket>> tickets;
@@ -481,7 +482,12 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
   std::unique_ptr<sdim_ctx> c(new sdim_ctx());
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
-  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int i = 0; i < 4; i++) {
+    hipStream_t st = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    c->streams.push_back(st);
+  }
+  c->stream = c->streams[0];
   if (prepare_kernels()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
   *out = c.release();
   return 0;
@@ -500,7 +506,7 @@ int sdim_ctx_destroy(sdim_ctx* c) {
     if (t->ev1) (void)hipEventDestroy(t->ev1);
   }
   for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
-  (void)hipStreamDestroy(c->stream);
+  for (auto st : c->streams) (void)hipStreamDestroy(st);
   delete c;
   return 0;
 }
@@ -551,14 +557,17 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   return 0;
 }
 
-int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) {
+}  // extern "C"
+
+// Lowers, plans, uploads and enqueues a submission on stream `stream_idx` (caller holds c->mu).
+static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, int stream_idx,
+                         uint64_t* ticket, uint64_t* device_bytes) {
   if (!c || (!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
   if (mode != SDIM_MODE_BATCHED && mode != SDIM_MODE_PLAIN) return fail(SDIM_EINVAL, "unknown mode");
-  std::lock_guard<std::mutex> lk(c->mu);
   HIP_OK(hipSetDevice(c->device));
   std::unique_ptr<Ticket> T(new Ticket());
   T->mode = mode;
-  T->stream = c->stream;
+  T->stream = c->streams[(size_t)stream_idx % c->streams.size()];
   T->jobs.resize((size_t)n_jobs);
   const uint32_t cap = (uint32_t)c->lds_bytes;
   T->classes = make_classes((int)c->hbm_threads, (int)c->hbm_per_cu);
@@ -756,8 +765,17 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
 
   T->id = c->next_ticket++;
   *ticket = T->id;
+  if (device_bytes) *device_bytes = T->arena.bytes + T->meta.bytes;
   c->tickets[T->id] = std::move(T);
   return 0;
+}
+
+extern "C" {
+
+int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return submit_locked(c, jobs, n_jobs, mode, 0, ticket, nullptr);
 }
 
 int sdim_wait(sdim_ctx* c, uint64_t ticket) {
@@ -929,6 +947,344 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
   if (elapsed_ms) *elapsed_ms = ms;
+  return 0;
+}
+
+}  // extern "C"
+
+// ==================================================================== serving loop
+// Continuous batching.  Requests (one circuit each) arrive through
+// sdim_server_submit at any time.  A dispatcher thread admits pending requests
+// in arrival order while their estimated device footprint fits the HBM budget,
+// launches the admitted ones as one submission on a free stream (up to
+// max_inflight submissions run concurrently on different streams, so new work
+// overlaps work already on the device), and on completion copies each
+// request's records to host memory and frees the device memory.  Every request
+// keeps its own Philox key, so its results do not depend on how it was batched.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  std::vector<int64_t> ir;
+  std::vector<int32_t> ch;
+  std::vector<double> pr;
+  sdim_job job{};
+  uint64_t footprint = 0;
+  double t_arrive = 0, t_launch = 0, t_done = 0;
+  int state = 0;                 // 0 pending, 1 in flight, 2 done, 3 failed
+  std::vector<uint8_t> records;  // MEASUREMENT_DTYPE bytes
+  int64_t n_records = 0;
+  uint64_t gate_shots = 0;
+  std::string error;
+};
+
+struct Batch {
+  uint64_t ticket = 0;
+  std::vector<uint64_t> reqs;
+  uint64_t bytes = 0;
+};
+
+uint64_t estimate_footprint(const sdim_job& j) {
+  const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d;
+  const uint64_t S = (uint64_t)std::min<int64_t>(j.shots, 4096);
+  const uint64_t items = ((uint64_t)j.shots + S - 1) / S;
+  int64_t n_meas = 0;
+  for (int64_t k = 0; k < j.n_ops; k++) {
+    const int64_t g = j.ir[3 * k];
+    if (g == 14 || g == 15 || g == 16) n_meas++;
+  }
+  const uint64_t tab = items * tableau_bytes(n, d, (uint32_t)S);
+  const uint64_t rec = (uint64_t)n_meas * (uint64_t)j.shots;
+  const uint64_t ops = (uint64_t)j.n_ops * 16 * 9 + 65536;   // worst-case primitive expansion
+  return round_block(tab + rec) + round_block(ops + 4096);
+}
+
+}  // namespace
+
+struct sdim_server {
+  sdim_ctx* c = nullptr;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<uint64_t> pending;
+  std::map<uint64_t, Request> reqs;
+  std::vector<Batch> inflight;            // dispatcher thread only
+  std::atomic<int64_t> n_inflight{0};
+  uint64_t budget = 0, used = 0;
+  int max_inflight = 4, max_batch = 256;
+  bool stop = false;
+  uint64_t next_id = 1;
+  uint64_t completed = 0;
+  double t_first_arrival = 0, t_last_done = 0;
+  uint64_t gate_shots_done = 0;
+  std::vector<double> latencies;
+};
+
+namespace {
+
+void serve_loop(sdim_server* s) {
+  sdim_ctx* c = s->c;
+  for (;;) {
+    // ---- completions
+    std::vector<Batch> still;
+    for (auto& b : s->inflight) {
+      bool done = false;
+      {
+        std::lock_guard<std::mutex> lk(c->mu);
+        Ticket* t = find_ticket(c, b.ticket);
+        if (t && hipEventQuery(t->ev1) == hipSuccess) {
+          (void)finish(c, t);
+          done = true;
+        }
+      }
+      if (!done) { still.push_back(b); continue; }
+      const double t_done = now_s();
+      std::vector<std::pair<uint64_t, std::vector<uint8_t>>> out;
+      {
+        std::lock_guard<std::mutex> lk(c->mu);
+        Ticket* t = find_ticket(c, b.ticket);
+        for (size_t k = 0; k < b.reqs.size(); k++) {
+          const JobPlan& jp = t->jobs[k];
+          std::vector<uint8_t> buf((size_t)(jp.n_meas * jp.shots) * 33);
+          // same record layout as sdim_fetch_records
+          for (int64_t ev = 0; ev < jp.n_meas; ev++) {
+            for (uint32_t i = 0; i < jp.n_items; i++) {
+              const ItemDesc& it = t->items[jp.first_item + i];
+              const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
+              for (uint32_t sh = 0; sh < it.S; sh++) {
+                const int64_t shot = (int64_t)it.shot0 + sh;
+                uint8_t* rec = buf.data() + 33 * (size_t)(ev * jp.shots + shot);
+                const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], val = r[sh] & 0x7F;
+                memcpy(rec, &q, 8); memcpy(rec + 8, &round, 8); memcpy(rec + 16, &shot, 8);
+                rec[24] = (r[sh] >> 7) & 1; memcpy(rec + 25, &val, 8);
+              }
+            }
+          }
+          out.emplace_back(b.reqs[k], std::move(buf));
+        }
+        auto it = c->tickets.find(b.ticket);
+        if (it != c->tickets.end()) {
+          ctx_free(c, it->second->arena);
+          ctx_free(c, it->second->meta);
+          for (auto& g : it->second->graph) if (g) (void)hipGraphExecDestroy(g);
+          if (it->second->ev0) (void)hipEventDestroy(it->second->ev0);
+          if (it->second->ev1) (void)hipEventDestroy(it->second->ev1);
+          c->tickets.erase(it);
+        }
+      }
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->used -= std::min(s->used, b.bytes);
+        for (auto& kv : out) {
+          Request& r = s->reqs[kv.first];
+          r.records = std::move(kv.second);
+          r.n_records = (int64_t)(r.records.size() / 33);
+          r.t_done = t_done;
+          r.state = 2;
+          s->completed++;
+          s->gate_shots_done += r.gate_shots;
+          s->latencies.push_back(t_done - r.t_arrive);
+          s->t_last_done = t_done;
+        }
+      }
+      s->cv.notify_all();
+    }
+    s->inflight.swap(still);
+    s->n_inflight = (int64_t)s->inflight.size();
+
+    // ---- admission: FIFO while the footprint fits the budget
+    std::vector<uint64_t> take;
+    uint64_t take_bytes = 0;
+    bool quit = false;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->stop && s->pending.empty() && s->inflight.empty()) quit = true;
+      if (!quit && (int)s->inflight.size() < s->max_inflight) {
+        while (!s->pending.empty() && (int)take.size() < s->max_batch) {
+          Request& r = s->reqs[s->pending.front()];
+          if (r.footprint > s->budget) {       // can never fit: fail it
+            r.state = 3;
+            r.error = "request larger than the HBM budget";
+            s->pending.pop_front();
+            continue;
+          }
+          if (s->used + take_bytes + r.footprint > s->budget) break;
+          take_bytes += r.footprint;
+          take.push_back(r.id);
+          s->pending.pop_front();
+        }
+        s->used += take_bytes;
+      }
+      if (!quit && take.empty()) {
+        // nothing to launch: wait for an arrival or poll completions again shortly
+        s->cv.wait_for(lk, std::chrono::microseconds(s->inflight.empty() ? 2000 : 50));
+      }
+    }
+    if (quit) break;
+    if (take.empty()) continue;
+    std::vector<sdim_job> jobs;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (uint64_t id : take) jobs.push_back(s->reqs[id].job);
+    }
+    uint64_t ticket = 0, dev_bytes = 0;
+    int rc;
+    {
+      std::lock_guard<std::mutex> lk(c->mu);
+      rc = submit_locked(c, jobs.data(), (int32_t)jobs.size(), SDIM_MODE_BATCHED,
+                         (int)(s->inflight.size() % c->streams.size()), &ticket, &dev_bytes);
+    }
+    const double t_launch = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc) {
+      for (uint64_t id : take) { s->reqs[id].state = 3; s->reqs[id].error = g_err; }
+      s->used -= std::min(s->used, take_bytes);
+      s->cv.notify_all();
+      continue;
+    }
+    for (uint64_t id : take) { s->reqs[id].state = 1; s->reqs[id].t_launch = t_launch; }
+    Batch b;
+    b.ticket = ticket;
+    b.reqs = take;
+    b.bytes = take_bytes;
+    s->inflight.push_back(b);
+    s->n_inflight = (int64_t)s->inflight.size();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdim_server_create(sdim_ctx* c, uint64_t hbm_budget_bytes, int32_t max_inflight, int32_t max_batch,
+                       sdim_server** out) {
+  if (!c || !out || max_inflight < 1 || max_batch < 1) return fail(SDIM_EINVAL, "bad server arguments");
+  sdim_server* s = new sdim_server();
+  s->c = c;
+  s->budget = hbm_budget_bytes;
+  s->max_inflight = std::min<int>(max_inflight, (int)c->streams.size());
+  s->max_batch = max_batch;
+  s->th = std::thread(serve_loop, s);
+  *out = s;
+  return 0;
+}
+
+int sdim_server_submit(sdim_server* s, const sdim_job* j, uint64_t* request_id) {
+  if (!s || !j || !request_id) return fail(SDIM_EINVAL, "null argument");
+  if (j->n < 1 || !supported_dim(j->d) || j->shots < 1 || j->n_ops < 0 || (j->n_ops && !j->ir))
+    return fail(SDIM_EINVAL, "invalid request");
+  for (int k = 0; k < 6; k++)
+    if (j->init[k]) return fail(SDIM_EINVAL, "serving requests start from |0...0>");
+  Request r;
+  r.ir.assign(j->ir, j->ir + 3 * j->n_ops);
+  if (j->noise_channel) r.ch.assign(j->noise_channel, j->noise_channel + j->n_ops);
+  if (j->noise_prob) r.pr.assign(j->noise_prob, j->noise_prob + j->n_ops);
+  r.job = *j;
+  r.job.ir = r.ir.data();
+  r.job.noise_channel = j->noise_channel ? r.ch.data() : nullptr;
+  r.job.noise_prob = j->noise_prob ? r.pr.data() : nullptr;
+  r.footprint = estimate_footprint(r.job);
+  uint64_t ops = 0;
+  for (int64_t k = 0; k < j->n_ops; k++) ops += (j->ir[3 * k] != 0);
+  r.gate_shots = ops * (uint64_t)j->shots;
+  r.t_arrive = now_s();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r.id = s->next_id++;
+    if (s->t_first_arrival == 0) s->t_first_arrival = r.t_arrive;
+    *request_id = r.id;
+    const uint64_t id = r.id;
+    s->reqs.emplace(id, std::move(r));
+    // vectors moved: re-point the job at the stored copies
+    Request& st = s->reqs[id];
+    st.job.ir = st.ir.data();
+    st.job.noise_channel = st.ch.empty() ? nullptr : st.ch.data();
+    st.job.noise_prob = st.pr.empty() ? nullptr : st.pr.data();
+    s->pending.push_back(id);
+  }
+  s->cv.notify_all();
+  return 0;
+}
+
+int sdim_server_poll(sdim_server* s, uint64_t id, int32_t* state) {
+  if (!s || !state) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(SDIM_ENOTFOUND, "unknown request");
+  *state = it->second.state;
+  return 0;
+}
+
+int sdim_server_wait(sdim_server* s, uint64_t id, double timeout_s) {
+  if (!s) return fail(SDIM_EINVAL, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  for (;;) {
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(SDIM_ENOTFOUND, "unknown request");
+    if (it->second.state == 2) return 0;
+    if (it->second.state == 3) return fail(SDIM_EINVAL, it->second.error);
+    if (s->cv.wait_until(lk, deadline) == std::cv_status::timeout) return fail(SDIM_EBUSY, "timeout");
+  }
+}
+
+int sdim_server_result(sdim_server* s, uint64_t id, void* out, int64_t capacity, int64_t* n_records,
+                       double* latency_s) {
+  if (!s) return fail(SDIM_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(SDIM_ENOTFOUND, "unknown request");
+  Request& r = it->second;
+  if (r.state == 3) return fail(SDIM_EINVAL, r.error);
+  if (r.state != 2) return fail(SDIM_EBUSY, "request not finished");
+  if (n_records) *n_records = r.n_records;
+  if (latency_s) *latency_s = r.t_done - r.t_arrive;
+  if (!out) return 0;                                  // size query
+  if (capacity < r.n_records) return fail(SDIM_EINVAL, "record buffer too small");
+  memcpy(out, r.records.data(), r.records.size());
+  s->reqs.erase(it);
+  return 0;
+}
+
+int sdim_server_stats(sdim_server* s, int64_t* completed, int64_t* pending, int64_t* inflight,
+                      uint64_t* gate_shots, double* span_s, double* p50_s, double* p99_s) {
+  if (!s) return fail(SDIM_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (completed) *completed = (int64_t)s->completed;
+  if (pending) *pending = (int64_t)s->pending.size();
+  if (inflight) *inflight = s->n_inflight.load();
+  if (gate_shots) *gate_shots = s->gate_shots_done;
+  if (span_s) *span_s = s->t_last_done > s->t_first_arrival ? s->t_last_done - s->t_first_arrival : 0.0;
+  std::vector<double> l = s->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double q) {
+    if (l.empty()) return 0.0;
+    size_t k = (size_t)std::ceil(q * (double)l.size()) - 1;
+    return l[std::min(k, l.size() - 1)];
+  };
+  if (p50_s) *p50_s = pct(0.50);
+  if (p99_s) *p99_s = pct(0.99);
+  return 0;
+}
+
+int sdim_server_destroy(sdim_server* s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->th.joinable()) s->th.join();
+  delete s;
   return 0;
 }
 
