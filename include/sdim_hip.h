@@ -83,6 +83,11 @@ const char* sdim_last_error(void);
  *   "allow_lds"       0 forces every tableau to stay in HBM
  *   "hbm_threads"     workgroup size of the launch class for small HBM-resident items (64..1024)
  *   "hbm_per_cu"      workgroups per CU of that class (its LDS budget is 160 KB / hbm_per_cu)
+ *   "gang_min_n"      circuits with at least this many qudits run on a gang of workgroups,
+ *                     each owning a range of generator columns (0 = never; default 1536)
+ *   "gang_chunk"      minimum bytes of a row each gang member owns (multiple of 16; default 64)
+ *   "gang_threads"    workgroup size of the gang kernel: 256, 512 (default) or 1024
+ *   "gang_max"        most workgroups per gang (0 = one per CU)
  */
 int sdim_set_option(sdim_ctx* ctx, const char* key, int64_t value);
 int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
@@ -123,6 +128,14 @@ int sdim_fetch_tableau(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t shot
  */
 int sdim_ticket_stats(sdim_ctx* ctx, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm,
                       uint64_t* bytes_lds, int64_t* n_items, int64_t* n_lds_items);
+
+/*
+ * How a ticket was laid out on the device, by key: "items", "lds_items" (items whose
+ * tableau stayed in LDS), "gang_items" (items run by gangs of workgroups), "gang_members"
+ * (largest gang), "gangs" (gangs launched), "launches" (persistent launches).
+ * Diagnostics only; no reference counterpart.
+ */
+int sdim_ticket_info(sdim_ctx* ctx, uint64_t ticket, const char* key, int64_t* value);
 
 /*
  * Raw device counters of the last run of a ticket: [0] tableau bytes of HBM-resident

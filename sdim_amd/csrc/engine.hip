@@ -95,9 +95,10 @@ struct Tab {
   uint8_t* X;
   uint8_t* Z;
   uint8_t* Ph;
-  uint32_t n, rs, S;
+  uint32_t n, rs, S;   // rs: row stride of X, Z and of the per-shot phase vectors
   uint32_t wofs;      // first column word of this view (column tiles of an HBM-resident tableau)
-  uint32_t rs_full;   // row stride of the whole tableau (noise snapshots are indexed by it)
+  uint32_t rs_full;   // width in bytes of the per-column LDS arrays (segment deltas, noise snapshots)
+  uint32_t nw;        // column words covered by the view (gate sweeps and phase passes)
 };
 
 SDIM_DEV uint32_t* row32(uint8_t* base, uint32_t r, uint32_t rs) { return (uint32_t*)(base + (size_t)r * rs); }
@@ -282,14 +283,14 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   const uint32_t nl = hdr.q, nn = hdr.t;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t tid = threadIdx.x;
-  const uint32_t G = 2 * n;
   const uint32_t ndesc = k + 1;
   k += 1 + nn;
   unsigned long long gb = 0;                   // bytes of the gate updates
 
   // noise draws of the segment: mask[j][S/32] bits + ab[j][S] bytes
   const uint32_t mw = (Sh + 31) / 32;
-  uint32_t* nmask = (uint32_t*)(S.noise + nn * 2 * rs);
+  const uint32_t lw = T.rs_full;                // width of the LDS per-column arrays
+  uint32_t* nmask = (uint32_t*)(S.noise + nn * 2 * lw);
   uint8_t* nab = (uint8_t*)(nmask + nn * mw);
   if (nn) {
     for (uint32_t i = tid; i < nn * mw; i += NT) nmask[i] = 0;
@@ -311,7 +312,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   uint32_t TW = rs;
   uint8_t* tile = nullptr;
   if (S.tiling) {
-    tile = S.noise + ((noise_bytes(nn, rs, Sh) + 15u) & ~15u);
+    tile = S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
     const int64_t left = (int64_t)(S.lds_end - tile);
     const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
     uint64_t tw = room / (2ull * n);
@@ -335,19 +336,20 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
       }
       __syncthreads();
     }
-    const uint32_t NWd = vw / CPWb;
+    const uint32_t NWd = (S.tiling && TW) ? vw / CPWb : T.nw;
     uint32_t kl = k;
   // word ownership: (word, slot) pairs when a row has at most NT words (the
   // slot partial deltas are reduced at the end); otherwise each thread owns
   // several whole words and adds each (layer, word) delta into DW directly.
   uint32_t NS, my_w, my_slot;
   const bool many = NWd > NT;
-  if (!many) { NS = NT / NWd; my_w = tid % NWd; my_slot = tid / NWd; }
+  if (NWd == 0) { NS = 1; my_w = 0; my_slot = 1; }           // a gang member without columns
+  else if (!many) { NS = NT / NWd; my_w = tid % NWd; my_slot = tid / NWd; }
   else { NS = 1; my_w = tid; my_slot = 0; }
   const bool active = my_slot < NS;
   if (many) {
     for (uint32_t w = tid; w < NWd; w += NT) {
-      if constexpr (BIN) { ((uint64_t*)S.DW)[V.wofs + w] = 0; ((uint64_t*)(S.DW + rs))[V.wofs + w] = 0; }
+      if constexpr (BIN) { ((uint64_t*)S.DW)[V.wofs + w] = 0; ((uint64_t*)(S.DW + lw))[V.wofs + w] = 0; }
       else ((uint32_t*)S.DW)[V.wofs + w] = 0;
     }
   }
@@ -399,7 +401,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
         if (many) {
           if constexpr (BIN) {
             uint64_t* p0 = (uint64_t*)S.DW + V.wofs + w;
-            uint64_t* p1 = (uint64_t*)(S.DW + rs) + V.wofs + w;
+            uint64_t* p1 = (uint64_t*)(S.DW + lw) + V.wofs + w;
             const uint64_t a0 = *p0, a1 = *p1;
             *p1 = a1 ^ b1 ^ (a0 & b0);
             *p0 = a0 ^ b0;
@@ -429,7 +431,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
           a1 ^= c1 ^ (a0 & c0); a0 ^= c0;
         }
         ((uint64_t*)S.DW)[V.wofs + w] = a0;
-        ((uint64_t*)(S.DW + rs))[V.wofs + w] = a1;
+        ((uint64_t*)(S.DW + lw))[V.wofs + w] = a1;
       }
     } else {
       if (active) S.red[tid] = dl;
@@ -458,7 +460,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   }
   // gate traffic lands in LDS when the tableau was staged through tiles
   if (S.tiling && TW) lbytes += gb; else bytes += gb;
-  const uint32_t NWd = BIN ? (G + 63) / 64 : (G + 3) / 4;
+  const uint32_t NWd = T.nw;
   SDIM_TADD(P, 7, tSeg);
   SDIM_TMARK(tPh);
   // per-shot phase pass: Ph_s += delta + noise terms of shot s
@@ -468,12 +470,12 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
       uint64_t* p0 = plane(T, s, 0) + w;
       uint64_t* p1 = plane(T, s, 1) + w;
       const uint64_t c0 = ((const uint64_t*)S.DW)[w];
-      uint64_t c1 = ((const uint64_t*)(S.DW + rs))[w];
+      uint64_t c1 = ((const uint64_t*)(S.DW + lw))[w];
       for (uint32_t j = 0; j < nn; j++) {
         if (nmask[j * mw + (s >> 5)] & (1u << (s & 31))) {
           const uint32_t ab = nab[j * Sh + s];
-          const uint8_t* snap = S.noise + j * 2 * rs;
-          const uint64_t sx = ((const uint64_t*)snap)[w], sz = ((const uint64_t*)(snap + rs))[w];
+          const uint8_t* snap = S.noise + j * 2 * lw;
+          const uint64_t sx = ((const uint64_t*)snap)[w], sz = ((const uint64_t*)(snap + lw))[w];
           c1 ^= (((ab >> 4) & 1) ? sx : 0ull) ^ ((ab & 1) ? sz : 0ull);
         }
       }
@@ -489,8 +491,8 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
       for (uint32_t j = 0; j < nn; j++) {
         if (nmask[j * mw + (s >> 5)] & (1u << (s & 31))) {
           const uint32_t ab = nab[j * Sh + s];
-          const uint8_t* snap = S.noise + j * 2 * rs;
-          const uint32_t sx = ((const uint32_t*)snap)[w], sz = ((const uint32_t*)(snap + rs))[w];
+          const uint8_t* snap = S.noise + j * 2 * lw;
+          const uint32_t sx = ((const uint32_t*)snap)[w], sz = ((const uint32_t*)(snap + lw))[w];
           c = addm<D>(c, subm<D>(mult<D>(S.mtab[(ab >> 4) & 15], sx), mult<D>(S.mtab[ab & 15], sz)));
         }
       }
@@ -911,6 +913,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     }
     SDIM_TADD(P, 6, tR);
   }
+  // the per-shot loops above read F, shot[] and the counters; the next op rewrites them
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ one item
@@ -927,7 +931,7 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   S.tiling = !res;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
   unsigned long long bytes = 0, lbytes = 0;
   SDIM_TMARK(tItem);
 
@@ -997,6 +1001,687 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
   }
 }
 
+// ------------------------------------------------------------------ gangs: one item on many workgroups
+// A tableau with thousands of qudits is processed by a gang of K workgroups.
+// Member m owns a contiguous range of column words (in every row of X and Z
+// and in every shot's phase vector), so gate segments need no coordination
+// (gates act on each generator column independently) and run with the
+// one-workgroup segment code on a view of the member's columns.  Measurements
+// synchronise the gang:
+//   random  A  own part of row q -> pivot candidate -> gang atomic-min  [barrier]
+//           B  each member gathers the pivot column and pivot phases    [barrier]
+//           C, D, E on own columns (the owners of p and n+p rewrite them)
+//   det     per row, the local scan over own stabilizer columns; per-row sums
+//           SX = sum f x, SZ = sum f z and per-shot partial phase sums are
+//           published                                                   [barrier]
+//           cross term sum_r sum_w SX_w[r] * sum_{v<w} SZ_v[r] over a row range,
+//           per-shot totals over a shot range                           [barrier]
+// The barrier is an arrival counter in global memory, published and observed
+// with agent-scope fences.  All members of a launch are resident together (the
+// grid is sized from the kernel's occupancy) and every wait is bounded: a
+// member that waits past a deadline raises the gang's error word, and every
+// member leaves the kernel when it sees it.
+struct GangSync {
+  uint32_t ctr;        // barrier arrivals
+  int err;             // nonzero: a member timed out
+  uint32_t item[2];    // item index broadcast (double-buffered)
+  int piv[2];          // pivot candidates (double-buffered by measurement count)
+  uint32_t pad[58];
+};
+static_assert(sizeof(GangSync) == 256, "GangSync is 256 bytes");
+
+constexpr uint64_t GANG_TIMEOUT = 1000000000ull;   // 10 s of the 100 MHz constant clock
+
+struct GangCtx {
+  GangSync* sy;
+  uint8_t* sxz;        // [n][Kp] bytes: SX | SZ << 4 of member v for row r
+  uint8_t* ps;         // [S][Kp] bytes: per-shot partial phase sums of member v
+  uint32_t* acc;       // [K] local scan sums
+  uint32_t* crs;       // [K] cross-term sums
+  uint32_t* tot;       // [S] per-shot totals
+  uint32_t K, Kp, me;
+  int* gerr;
+  uint32_t target;     // arrivals expected at the next barrier
+};
+
+SDIM_DEV bool gang_barrier(GangCtx& g, int* s_ok) {
+  __syncthreads();
+  g.target += g.K;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __threadfence();
+    __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(&g.sy->ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g.target) {
+      if (__hip_atomic_load(&g.sy->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+      if (wall_clock64() - t0 > GANG_TIMEOUT) {
+        __hip_atomic_store(&g.sy->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(g.gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+    *s_ok = ok;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// identity on the member's columns: V points at byte c_lo of every row, cb own bytes
+template <int D, int NT>
+SDIM_DEV void gang_init_identity(const Tab& V, uint32_t c_lo, uint32_t cb) {
+  const uint32_t n = V.n, rs = V.rs;
+  const uint32_t rows = 2 * n + (D == 2 ? 2 : 1) * V.S;   // X, Z and phase rows are consecutive
+  const uint32_t q16 = cb / 16;
+  for (uint32_t i = threadIdx.x; i < rows * q16; i += NT) {
+    const uint32_t r = i / q16, c = i - r * q16;
+    *(uint4*)(V.X + (size_t)r * rs + 16 * c) = make_uint4(0, 0, 0, 0);
+  }
+  __syncthreads();
+  for (uint32_t r = threadIdx.x; r < n; r += NT) {
+    if constexpr (D == 2) {
+      const uint32_t g = n + r;
+      if (g >= c_lo * 8 && g < (c_lo + cb) * 8) row64(V.X, r, rs)[(g >> 6) - c_lo / 8] |= 1ull << (g & 63);
+      if (r >= c_lo * 8 && r < (c_lo + cb) * 8) row64(V.Z, r, rs)[(r >> 6) - c_lo / 8] |= 1ull << (r & 63);
+    } else {
+      if (n + r >= c_lo && n + r < c_lo + cb) V.X[(size_t)r * rs + n + r - c_lo] = 1;
+      if (r >= c_lo && r < c_lo + cb) V.Z[(size_t)r * rs + r - c_lo] = 1;
+    }
+  }
+  __syncthreads();
+}
+
+// Cross term of one row over the K members' published bytes b_v = SX | SZ << 4:
+// sum_v SX_v * (sum_{u<v} SZ_u), mod d (parity for d = 2).
+template <int D>
+SDIM_DEV uint32_t gang_row_cross(const uint8_t* row, uint32_t K) {
+  uint32_t pre = 0, c = 0;
+  for (uint32_t v0 = 0; v0 < K; v0 += 16) {
+    const uint4 q = *(const uint4*)(row + v0);
+    const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+    #pragma unroll
+    for (int j = 0; j < 16; j++) {
+      if (v0 + (uint32_t)j < K) {
+        const uint32_t b = (wd[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        if constexpr (D == 2) { c ^= (b & 1u) & pre; pre ^= (b >> 4) & 1u; }
+        else { c += (b & 15u) * pre; pre = (pre + (b >> 4)) % D; }
+      }
+    }
+    if constexpr (D != 2) c %= D;
+  }
+  return c;
+}
+
+// plain sum of the K bytes of one row
+template <int D>
+SDIM_DEV uint32_t gang_row_sum(const uint8_t* row, uint32_t K) {
+  uint32_t t = 0;
+  for (uint32_t v0 = 0; v0 < K; v0 += 16) {
+    const uint4 q = *(const uint4*)(row + v0);
+    const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+    #pragma unroll
+    for (int j = 0; j < 16; j++)
+      if (v0 + (uint32_t)j < K) t += (wd[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+  }
+  return t;
+}
+
+// Measurement of qudit q by one member.  Tg: the whole tableau; V: the member's
+// columns (words w0 .. w0 + V.nw).  Returns false when the gang was aborted.
+template <int D, int NT>
+SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q, uint32_t slot, uint32_t src,
+                           bool reset, uint32_t mk, const Scr& S, const ItemDesc& it, const KParams& P,
+                           GangCtx& Gc, int* s_ok, unsigned long long& bytes) {
+  constexpr bool BIN = (D == 2);
+  constexpr uint32_t CPW = BIN ? 64 : 4;
+  constexpr int RB = 8;     // rows per load batch in the sweep
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n = Tg.n, rs = Tg.rs, Sh = Tg.S;
+  const uint32_t nw = V.nw;
+  const uint32_t nsw = (n + CPW - 1) / CPW;
+  const uint32_t c_lo = w0 * CPW, c_hi = (w0 + nw) * CPW;   // own columns
+  int* M = S.misc + 16 * (mk & 1);
+  int* Mn = S.misc + 16 * ((mk + 1) & 1);
+  uint8_t* rec = P.arena + it.rec_off + (size_t)slot * Sh;
+  const bool lead = Gc.me == 0;
+
+  // ---- A: own words of row q -> pivot candidate, f, words with f != 0
+  for (uint32_t w = tid; w < nw; w += NT) {
+    S.DOT[2 * w] = 0;
+    S.DOT[2 * w + 1] = 0;
+    const uint32_t gw = w0 + w;
+    if constexpr (BIN) {
+      const uint64_t v = row64(V.X, q, rs)[w];
+      ((uint64_t*)S.F)[w] = v;
+      if (v) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
+      if (gw < nsw) {
+        uint64_t vs = v;
+        const uint32_t lim = n - 64 * gw;
+        if (lim < 64) vs &= (1ull << lim) - 1;
+        if (vs) atomicMin(&M[0], (int)(64 * gw + __builtin_ctzll(vs)));
+      }
+      bytes += 8;
+    } else {
+      const uint32_t v = row32(V.X, q, rs)[w];
+      const uint32_t f = negm<D>(v);
+      ((uint32_t*)S.F)[w] = f;
+      if (f) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
+      if (gw < nsw) {
+        uint32_t vs = v;
+        const uint32_t lim = n - 4 * gw;
+        if (lim < 4) vs &= (1u << (8 * lim)) - 1;
+        if (vs) atomicMin(&M[0], (int)(4 * gw + (__builtin_ctz(vs) >> 3)));
+      }
+      bytes += 4;
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && M[0] != INT_MAX)
+    __hip_atomic_fetch_min(&Gc.sy->piv[mk & 1], M[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!gang_barrier(Gc, s_ok)) return false;                                // G1
+  const int piv = __hip_atomic_load(&Gc.sy->piv[mk & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    Mn[0] = INT_MAX; Mn[1] = 0; Mn[2] = 0; Mn[3] = 0;
+    // the other slot is next written in measurement mk + 1, after this one's last barrier
+    if (lead) __hip_atomic_store(&Gc.sy->piv[(mk + 1) & 1], INT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  if (piv != INT_MAX) {
+    const uint32_t p = (uint32_t)piv;
+    const bool own_p = p >= c_lo && p < c_hi;
+    const bool own_g = n + p >= c_lo && n + p < c_hi;
+    // ---- B: every member gathers column p (whole tableau), touched rows, <x_p, z_p>, pivot phases
+    for (uint32_t r = tid; r < n; r += NT) {
+      uint32_t xr, zr;
+      if constexpr (BIN) {
+        xr = (uint32_t)((row64(Tg.X, r, rs)[p >> 6] >> (p & 63)) & 1);
+        zr = (uint32_t)((row64(Tg.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+        if (own_p) bytes += 16;
+      } else {
+        xr = Tg.X[(size_t)r * rs + p];
+        zr = Tg.Z[(size_t)r * rs + p];
+        if (own_p) bytes += 2;
+      }
+      S.xp[r] = (uint8_t)xr;
+      S.zp[r] = (uint8_t)zr;
+      if (xr | zr) {
+        S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
+        if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
+      }
+    }
+    if (tid == 0) {
+      if constexpr (BIN) {
+        if (own_p) ((uint64_t*)S.F)[(p >> 6) - w0] &= ~(1ull << (p & 63));
+        if (own_g) ((uint64_t*)S.F)[((n + p) >> 6) - w0] &= ~(1ull << ((n + p) & 63));
+      } else {
+        if (own_p) S.F[p - c_lo] = 0;
+        if (own_g) S.F[n + p - c_lo] = 0;
+      }
+    }
+    for (uint32_t s = tid; s < Sh; s += NT) {
+      if constexpr (BIN) {
+        const uint32_t v0 = (uint32_t)((plane(Tg, s, 0)[p >> 6] >> (p & 63)) & 1);
+        const uint32_t v1 = (uint32_t)((plane(Tg, s, 1)[p >> 6] >> (p & 63)) & 1);
+        S.shot[s] = v0 | (v1 << 1);
+      } else {
+        S.shot[s] = phase_vec(Tg, s)[p];
+      }
+    }
+    // every member holds column p before its owner rewrites it
+    if (!gang_barrier(Gc, s_ok)) return false;                              // G2
+    const uint32_t nT = (uint32_t)M[1], nF = (uint32_t)M[2];
+    uint32_t e = 1, xz = 0, pinc = 0;
+    if constexpr (!BIN) {
+      const uint32_t xv = S.xp[q];
+      #pragma unroll
+      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
+      const uint32_t xz_old = (uint32_t)M[3] % D;
+      xz = ((e * e) % D) * xz_old % D;
+      pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
+    }
+
+    // ---- C: sweep (touched rows) x (own words with f != 0)
+    if (nT && nF) {
+      uint32_t NRG, j0, g0, jstep;
+      if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
+      else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
+      if (g0 < NRG) {
+        for (uint32_t j = j0; j < nF; j += jstep) {
+          const uint32_t w = S.FL[j];
+          if constexpr (BIN) {
+            const uint64_t f = ((const uint64_t*)S.F)[w];
+            uint64_t dacc = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB];
+              uint64_t xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
+                  zv[b] = row64(V.Z, rr[b], rs)[w];
+                  xv[b] = sxv[b] ? row64(V.X, rr[b], rs)[w] : 0ull;
+                }
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  if (sxv[b]) { dacc ^= zv[b]; row64(V.X, rr[b], rs)[w] = xv[b] ^ f; bytes += 16; }
+                  if (szv[b]) { row64(V.Z, rr[b], rs)[w] = zv[b] ^ f; bytes += 8; }
+                  bytes += 8;
+                }
+              }
+            }
+            if (dacc) {
+              atomicXor(&S.DOT[2 * w], (uint32_t)dacc);
+              atomicXor(&S.DOT[2 * w + 1], (uint32_t)(dacc >> 32));
+            }
+          } else {
+            const uint32_t f = ((const uint32_t*)S.F)[w];
+            uint32_t de = 0, dO = 0, cnt = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB], xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = (e * S.xp[rr[b]]) % D;
+                  szv[b] = (e * S.zp[rr[b]]) % D;
+                  zv[b] = row32(V.Z, rr[b], rs)[w];
+                  xv[b] = sxv[b] ? row32(V.X, rr[b], rs)[w] : 0u;
+                }
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  if (sxv[b]) {
+                    const MulTab& t = S.mtab[sxv[b]];
+                    const uint32_t pz = mult<D>(t, zv[b]);
+                    de += pz & 0x00FF00FFu;
+                    dO += (pz >> 8) & 0x00FF00FFu;
+                    row32(V.X, rr[b], rs)[w] = addm<D>(xv[b], mult<D>(t, f));
+                    bytes += 8;
+                  }
+                  if (szv[b]) { row32(V.Z, rr[b], rs)[w] = addm<D>(zv[b], mult<D>(S.mtab[szv[b]], f)); bytes += 4; }
+                  bytes += 4;
+                }
+              }
+              if (++cnt == 512) { de = fold16<D>(de); dO = fold16<D>(dO); cnt = 0; }
+            }
+            de = fold16<D>(de);
+            dO = fold16<D>(dO);
+            if (de) atomicAdd(&S.DOT[2 * w], de);
+            if (dO) atomicAdd(&S.DOT[2 * w + 1], dO);
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- D: own parts of columns p and n+p; per-shot phases over own words with f != 0
+    if (own_p || own_g) {
+      for (uint32_t r = tid; r < n; r += NT) {
+        if constexpr (BIN) {
+          const uint64_t xv = S.xp[r], zv = S.zp[r];
+          uint64_t* xr = row64(V.X, r, rs);
+          uint64_t* zr = row64(V.Z, r, rs);
+          const uint32_t g = n + p;
+          if (own_p) {
+            const uint32_t wl = (p >> 6) - w0;
+            uint64_t x = xr[wl] & ~(1ull << (p & 63));
+            uint64_t z = (zr[wl] & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
+            if ((g >> 6) == (p >> 6)) {
+              x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
+              z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
+            }
+            xr[wl] = x; zr[wl] = z;
+          }
+          if (own_g && (g >> 6) != (p >> 6)) {
+            const uint32_t wl = (g >> 6) - w0;
+            xr[wl] = (xr[wl] & ~(1ull << (g & 63))) | (xv << (g & 63));
+            zr[wl] = (zr[wl] & ~(1ull << (g & 63))) | (zv << (g & 63));
+          }
+          if (own_p) bytes += 16;
+          if (own_g && (g >> 6) != (p >> 6)) bytes += 16;
+        } else {
+          if (own_p) {
+            V.X[(size_t)r * rs + p - c_lo] = 0;
+            V.Z[(size_t)r * rs + p - c_lo] = (uint8_t)(r == q);
+            bytes += 2;
+          }
+          if (own_g) {
+            V.X[(size_t)r * rs + n + p - c_lo] = (uint8_t)((e * S.xp[r]) % D);
+            V.Z[(size_t)r * rs + n + p - c_lo] = (uint8_t)((e * S.zp[r]) % D);
+            bytes += 2;
+          }
+        }
+      }
+    }
+    for (uint32_t i = tid; i < Sh * nF; i += NT) {
+      const uint32_t s = i / nF, j = i - s * nF;
+      const uint32_t w = S.FL[j];
+      if constexpr (BIN) {
+        const uint32_t k2 = S.shot[s];
+        const uint64_t f = ((const uint64_t*)S.F)[w];
+        const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
+        uint64_t* p0 = plane(V, s, 0) + w;
+        uint64_t* p1 = plane(V, s, 1) + w;
+        uint64_t a0 = *p0, a1 = *p1;
+        const uint64_t k0m = (k2 & 1) ? f : 0ull, k1m = (k2 & 2) ? f : 0ull;
+        a1 ^= k1m ^ (a0 & k0m);
+        a0 ^= k0m;
+        a1 ^= cdot;
+        *p0 = a0; *p1 = a1;
+        bytes += 32;
+      } else {
+        const uint32_t f = ((const uint32_t*)S.F)[w];
+        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
+        uint32_t c = 0;
+        #pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+          const uint32_t fb = byte_of(f, kb);
+          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
+          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
+        }
+        const uint32_t php = (e * S.shot[s] + pinc) % D;
+        uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
+        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
+        bytes += 8;
+      }
+    }
+    __syncthreads();
+    // ---- E: outcomes; ph_{n+p} <- ph_p, ph_p <- -m po on the owners' columns
+    for (uint32_t s = tid; s < Sh; s += NT) {
+      const U64x4 o = draw(it.seed, it.stream, src, it.shot0 + s);
+      const uint32_t m = outcome_of(o, D);
+      if constexpr (BIN) {
+        const uint32_t k2 = S.shot[s];
+        const uint32_t g = n + p;
+        uint64_t* p0 = plane(V, s, 0);
+        uint64_t* p1 = plane(V, s, 1);
+        if (own_g) {
+          const uint32_t wl = (g >> 6) - w0;
+          p0[wl] = (p0[wl] & ~(1ull << (g & 63))) | ((uint64_t)(k2 & 1) << (g & 63));
+          p1[wl] = (p1[wl] & ~(1ull << (g & 63))) | ((uint64_t)(k2 >> 1) << (g & 63));
+        }
+        if (own_p) {
+          const uint32_t wl = (p >> 6) - w0;
+          p0[wl] &= ~(1ull << (p & 63));
+          p1[wl] = (p1[wl] & ~(1ull << (p & 63))) | ((uint64_t)(m & 1) << (p & 63));
+        }
+      } else {
+        uint8_t* ph = phase_vec(V, s);
+        if (own_g) ph[n + p - c_lo] = (uint8_t)((e * S.shot[s] + pinc) % D);
+        if (own_p) ph[p - c_lo] = (uint8_t)((D - m) % D);
+      }
+      S.shot[Sh + s] = m;
+      if (lead) rec[s] = (uint8_t)m;
+    }
+  } else {
+    // ---- deterministic outcome: prod_i S_i^{f_i}, f_i = dx[q, i] = X[q][n + i]
+    const uint32_t ns_own = (w0 < nsw) ? ((nsw - w0) < nw ? (nsw - w0) : nw) : 0u;
+    for (uint32_t w = tid; w < ns_own; w += NT) {
+      const uint32_t gw = w0 + w;
+      if constexpr (BIN) {
+        const uint64_t* rq = row64(Tg.X, q, rs);
+        const uint32_t g = n + 64 * gw;
+        const uint64_t lo = rq[g >> 6] >> (g & 63);
+        const uint64_t hi = ((g & 63) && (g >> 6) + 1 < rs / 8) ? rq[(g >> 6) + 1] << (64 - (g & 63)) : 0ull;
+        uint64_t f = lo | hi;
+        const uint32_t lim = n - 64 * gw;
+        if (lim < 64) f &= (1ull << lim) - 1;
+        ((uint64_t*)S.F)[w] = f;
+        bytes += 16;
+      } else {
+        uint32_t f = 0;
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const uint32_t c = 4 * gw + b;
+          if (c < n) f |= (uint32_t)Tg.X[(size_t)q * rs + n + c] << (8 * b);
+        }
+        ((uint32_t*)S.F)[w] = f;
+        bytes += 4;
+      }
+    }
+    __syncthreads();
+    // local scan of every row over own stabilizer columns; publish SX, SZ of the row
+    uint32_t part = 0;
+    for (uint32_t r = tid; r < n; r += NT) {
+      uint32_t sx = 0, sz = 0;
+      if constexpr (BIN) {
+        const uint64_t* xr = row64(V.X, r, rs);
+        const uint64_t* zr = row64(V.Z, r, rs);
+        uint64_t carry = 0, par = 0, pa = 0;
+        for (uint32_t w = 0; w < ns_own; w++) {
+          const uint64_t f = ((const uint64_t*)S.F)[w];
+          if (!f) continue;
+          const uint64_t a = xr[w] & f, b = zr[w] & f;
+          uint64_t y = b;
+          y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
+          const uint64_t pre = (y << 1) ^ (carry ? ~0ull : 0ull);
+          par ^= (uint64_t)(__builtin_popcountll(a & pre) & 1);
+          carry ^= (uint64_t)(__builtin_popcountll(b) & 1);
+          pa ^= (uint64_t)(__builtin_popcountll(a) & 1);
+          bytes += 16;
+        }
+        part += (uint32_t)par;
+        sx = (uint32_t)pa; sz = (uint32_t)carry;
+      } else {
+        const uint32_t* xr = row32(V.X, r, rs);
+        const uint32_t* zr = row32(V.Z, r, rs);
+        uint32_t U = 0, acc = 0, ax = 0;
+        for (uint32_t w = 0; w < ns_own; w++) {
+          const uint32_t f = ((const uint32_t*)S.F)[w];
+          if (!f) continue;
+          const uint32_t x = xr[w], z = zr[w];
+          #pragma unroll
+          for (int b = 0; b < 4; b++) {
+            const uint32_t fb = byte_of(f, b);
+            const uint32_t xb = byte_of(x, b), zb = byte_of(z, b);
+            acc += fb * xb * U + ((fb * (fb - 1) / 2) % D) * xb * zb;
+            U = (U + fb * zb) % D;
+            ax += fb * xb;
+          }
+          bytes += 8;
+        }
+        part += acc % D;
+        sx = ax % D; sz = U;
+      }
+      Gc.sxz[(size_t)r * Gc.Kp + Gc.me] = (uint8_t)(sx | (sz << 4));
+    }
+    const uint32_t loc = BIN ? (block_sum<NT>(part, S.red) & 1) : (block_sum<NT>(part % D, S.red) % D);
+    if (tid == 0) Gc.acc[Gc.me] = loc;
+    // per-shot partial sums of f_i ph_i over own stabilizer columns
+    for (uint32_t s = tid; s < Sh; s += NT) {
+      uint32_t v;
+      if constexpr (BIN) {
+        uint32_t c0 = 0, c1 = 0;
+        for (uint32_t w = 0; w < ns_own; w++) {
+          const uint64_t f = ((const uint64_t*)S.F)[w];
+          if (!f) continue;
+          c0 += __builtin_popcountll(plane(V, s, 0)[w] & f);
+          c1 += __builtin_popcountll(plane(V, s, 1)[w] & f);
+          bytes += 16;
+        }
+        v = (c0 + 2 * c1) & 3;
+      } else {
+        const uint32_t* ph = (const uint32_t*)phase_vec(V, s);
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < ns_own; w++) {
+          const uint32_t f = ((const uint32_t*)S.F)[w];
+          if (!f) continue;
+          const uint32_t pv = ph[w];
+          #pragma unroll
+          for (int b = 0; b < 4; b++) acc += byte_of(f, b) * byte_of(pv, b);
+          bytes += 4;
+        }
+        v = acc % D;
+      }
+      Gc.ps[(size_t)s * Gc.Kp + Gc.me] = (uint8_t)v;
+    }
+    if (!gang_barrier(Gc, s_ok)) return false;                              // G2
+    // cross term over this member's row range, per-shot totals over its shot range
+    const uint32_t r0 = (uint32_t)(((uint64_t)n * Gc.me) / Gc.K), r1 = (uint32_t)(((uint64_t)n * (Gc.me + 1)) / Gc.K);
+    uint32_t cr = 0;
+    for (uint32_t r = r0 + tid; r < r1; r += NT) cr += gang_row_cross<D>(Gc.sxz + (size_t)r * Gc.Kp, Gc.K);
+    const uint32_t crs = BIN ? (block_sum<NT>(cr, S.red) & 1) : (block_sum<NT>(cr, S.red) % D);
+    if (tid == 0) Gc.crs[Gc.me] = crs;
+    const uint32_t s0 = (uint32_t)(((uint64_t)Sh * Gc.me) / Gc.K), s1 = (uint32_t)(((uint64_t)Sh * (Gc.me + 1)) / Gc.K);
+    for (uint32_t s = s0 + tid; s < s1; s += NT) {
+      const uint32_t t = gang_row_sum<D>(Gc.ps + (size_t)s * Gc.Kp, Gc.K);
+      Gc.tot[s] = BIN ? (t & 3) : (t % D);
+    }
+    if (!gang_barrier(Gc, s_ok)) return false;                              // G3
+    uint32_t kk = 0;
+    for (uint32_t v = tid; v < Gc.K; v += NT) kk += Gc.acc[v] + Gc.crs[v];
+    const uint32_t Kc = BIN ? (block_sum<NT>(kk, S.red) & 1) : (block_sum<NT>(kk, S.red) % D);
+    for (uint32_t s = tid; s < Sh; s += NT) {
+      const uint32_t t = Gc.tot[s];
+      uint32_t m;
+      if constexpr (BIN) m = ((((t + 2 * Kc) & 3) + 1) >> 1) & 1;
+      else m = (D - (t + Kc) % D) % D;
+      S.shot[Sh + s] = m;
+      if (lead) rec[s] = (uint8_t)(m | 0x80u);
+    }
+  }
+
+  if (reset) {
+    __syncthreads();
+    // X^k with k = (-m) mod d on own columns: phase -= k po z[q, .]
+    for (uint32_t i = tid; i < Sh * nw; i += NT) {
+      const uint32_t s = i / nw, w = i - s * nw;
+      const uint32_t m = S.shot[Sh + s];
+      if constexpr (BIN) {
+        if (m & 1) {
+          const uint64_t z = row64(V.Z, q, rs)[w];
+          if (z) { plane(V, s, 1)[w] ^= z; bytes += 24; }
+        }
+      } else {
+        const uint32_t kk = (D - m) % D;
+        if (kk) {
+          const uint32_t z = row32(V.Z, q, rs)[w];
+          if (z) {
+            uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
+            *ph = subm<D>(*ph, mult<D>(S.mtab[kk], z));
+            bytes += 12;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+// One item on one member of a gang.  Returns false when the gang was aborted.
+template <int D, int NT>
+SDIM_DEV bool gang_item(const ItemDesc& it, const KParams& P, GangCtx& Gc, uint8_t* lds, int* s_ok, uint32_t& mk) {
+  constexpr uint32_t CPWb = (D == 2) ? 8 : 4;
+  const uint32_t n = it.n, rs = it.rs, Sh = it.S;
+  const uint32_t NWall = rs / CPWb;
+  const uint32_t cwn = gang_words(NWall, Gc.K, CPWb);
+  const uint32_t w0 = (Gc.me * cwn < NWall) ? Gc.me * cwn : NWall;
+  const uint32_t w1 = (w0 + cwn < NWall) ? w0 + cwn : NWall;
+  uint8_t* tab = P.arena + it.tab_off;
+  Tab Tg;
+  Tg.X = tab; Tg.Z = tab + (size_t)n * rs; Tg.Ph = tab + 2 * (size_t)n * rs;
+  Tg.n = n; Tg.rs = rs; Tg.S = Sh; Tg.wofs = 0; Tg.rs_full = rs; Tg.nw = NWall;
+  Tab V = Tg;
+  V.X += (size_t)w0 * CPWb; V.Z += (size_t)w0 * CPWb; V.Ph += (size_t)w0 * CPWb;
+  V.rs_full = cwn * CPWb; V.nw = w1 - w0;
+  Scr S = carve<NT>(lds, n, cwn * CPWb, Sh);
+  S.lds_end = lds + P.lds_bytes;
+  S.tiling = false;
+  unsigned long long bytes = 0, lbytes = 0;
+  if (threadIdx.x < 16) S.mtab[threadIdx.x] = make_tab<D>(threadIdx.x < D ? threadIdx.x : 0);
+  if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
+  if (!(it.flags & ITEM_INIT_GIVEN)) {
+    gang_init_identity<D, NT>(V, w0 * CPWb, (w1 - w0) * CPWb);
+    if (threadIdx.x == 0) bytes += (uint64_t)(2 * n + (D == 2 ? 2 : 1) * Sh) * (w1 - w0) * CPWb;
+  } else {
+    __syncthreads();
+  }
+  const DevOp* ops = P.ops + it.ops_off;
+  uint32_t k = 0;
+  bool ok = true;
+  while (k < it.n_ops) {
+    const DevOp o = ops[k];
+    const uint32_t kind = op_kind(o);
+    if (kind == OP_SEG) {
+      k = run_segment<D, NT>(ops, k, V, S, it, P, bytes, lbytes);
+    } else if (kind == OP_MEAS || kind == OP_RESET) {
+      ok = gang_measure<D, NT>(Tg, V, w0, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, Gc, s_ok, bytes);
+      if (!ok) break;
+      k++;
+    } else {
+      break;
+    }
+  }
+  const unsigned long long tot = block_sum64<NT>(bytes + lbytes, S.red);
+  if (threadIdx.x == 0) atomicAdd(&P.counters[0], tot);
+  return ok;
+}
+
+// Persistent gang kernel: gang g = blockIdx.x / K pulls items from P.queue.
+template <int NT>
+__global__ __launch_bounds__(NT) void gang_kernel(KParams P, GangParams G) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ int s_ok;
+  __shared__ uint32_t s_item;
+  const uint32_t g = blockIdx.x / G.K;
+  GangCtx Gc;
+  Gc.sy = G.sync + g;
+  uint8_t* base = G.scr + (size_t)g * G.scr_stride;
+  Gc.Kp = gang_kp(G.K);
+  Gc.sxz = base;
+  Gc.ps = base + gang_align(G.n_max * (uint64_t)Gc.Kp);
+  Gc.acc = (uint32_t*)(Gc.ps + gang_align(G.S_max * (uint64_t)Gc.Kp));
+  Gc.crs = Gc.acc + G.K;
+  Gc.tot = Gc.crs + G.K;
+  Gc.K = G.K;
+  Gc.me = blockIdx.x % G.K;
+  Gc.gerr = G.err;
+  Gc.target = 0;
+  uint32_t mk = 0, round = 0;
+  for (;;) {
+    if (Gc.me == 0 && threadIdx.x == 0)
+      __hip_atomic_store(&Gc.sy->item[round & 1], atomicAdd(P.queue, 1u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!gang_barrier(Gc, &s_ok)) return;
+    if (threadIdx.x == 0) s_item = __hip_atomic_load(&Gc.sy->item[round & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t idx = s_item;
+    round++;
+    if (idx >= P.n_items) return;
+    const ItemDesc it = P.items[P.order[idx]];
+    bool ok = false;
+    switch (it.d) {
+      case 2: ok = gang_item<2, NT>(it, P, Gc, lds, &s_ok, mk); break;
+      case 3: ok = gang_item<3, NT>(it, P, Gc, lds, &s_ok, mk); break;
+      case 5: ok = gang_item<5, NT>(it, P, Gc, lds, &s_ok, mk); break;
+      case 7: ok = gang_item<7, NT>(it, P, Gc, lds, &s_ok, mk); break;
+      case 11: ok = gang_item<11, NT>(it, P, Gc, lds, &s_ok, mk); break;
+      default: break;
+    }
+    if (!ok) return;
+  }
+}
+
+__global__ void gang_reset_kernel(GangSync* sync, uint32_t n_gangs, int* err) {
+  for (uint32_t i = threadIdx.x; i < n_gangs; i += blockDim.x) {
+    sync[i].ctr = 0;
+    sync[i].err = 0;
+    sync[i].item[0] = 0; sync[i].item[1] = 0;
+    sync[i].piv[0] = INT_MAX; sync[i].piv[1] = INT_MAX;
+  }
+  if (threadIdx.x == 0) *err = 0;
+}
+
 // ------------------------------------------------------------------ plain path: one launch per op
 // Every item of one circuit (its shot groups) is processed by the same launch;
 // the tableau stays in HBM.  Gates update the phases immediately.
@@ -1006,7 +1691,7 @@ __global__ __launch_bounds__(256) void plain_init_kernel(KParams P) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)it.n * it.rs; T.Ph = tab + 2 * (size_t)it.n * it.rs;
-  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs;
+  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs; T.nw = it.rs / (D == 2 ? 8 : 4);
   init_identity<D, 256>(T, tableau_bytes(it.n, D, it.S));
 }
 
@@ -1024,7 +1709,7 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
   Scr S;
   S.mtab = mt;
   S.tiling = false;
@@ -1086,7 +1771,7 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
   const Scr S = carve<NT>(lds, n, rs, Sh);
   if (threadIdx.x < 16) S.mtab[threadIdx.x] = make_tab<D>(threadIdx.x < D ? threadIdx.x : 0);
   if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
@@ -1101,7 +1786,7 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
 namespace sdim {
 
 __global__ void reset_kernel(uint32_t* queue, unsigned long long* counters) {
-  if (threadIdx.x < 8) queue[threadIdx.x] = 0;
+  if (threadIdx.x < SDIM_MAX_CLASSES) queue[threadIdx.x] = 0;
   if (threadIdx.x < 2 + SDIM_NUM_TIMERS) counters[threadIdx.x] = 0;
 }
 
@@ -1112,6 +1797,9 @@ int prepare_kernels() {
   if (hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)gang_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)gang_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void*)gang_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
 #define SDIM_SET_PLAIN(D) \
   if (hipFuncSetAttribute((const void*)plain_meas_kernel<D, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   SDIM_SET_PLAIN(2) SDIM_SET_PLAIN(3) SDIM_SET_PLAIN(5) SDIM_SET_PLAIN(7) SDIM_SET_PLAIN(11)
@@ -1132,6 +1820,19 @@ int launch_batch(const KParams& P, int nt, int grid, hipStream_t st) {
     case 256: hipLaunchKernelGGL(batch_kernel<256>, dim3(grid), dim3(256), lds, st, P); break;
     case 512: hipLaunchKernelGGL(batch_kernel<512>, dim3(grid), dim3(512), lds, st, P); break;
     case 1024: hipLaunchKernelGGL(batch_kernel<1024>, dim3(grid), dim3(1024), lds, st, P); break;
+    default: return -2;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_gang(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st) {
+  const size_t lds = P.lds_bytes;
+  const uint32_t n_gangs = (uint32_t)grid / G.K;
+  hipLaunchKernelGGL(gang_reset_kernel, dim3(1), dim3(256), 0, st, G.sync, n_gangs, G.err);
+  switch (nt) {
+    case 256: hipLaunchKernelGGL(gang_kernel<256>, dim3(grid), dim3(256), lds, st, P, G); break;
+    case 512: hipLaunchKernelGGL(gang_kernel<512>, dim3(grid), dim3(512), lds, st, P, G); break;
+    case 1024: hipLaunchKernelGGL(gang_kernel<1024>, dim3(grid), dim3(1024), lds, st, P, G); break;
     default: return -2;
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;

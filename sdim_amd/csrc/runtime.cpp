@@ -86,6 +86,7 @@ struct LaunchClass {
   int nt;          // threads per workgroup
   uint32_t lds;    // dynamic LDS bytes per workgroup
   int per_cu;      // workgroups resident per CU
+  uint32_t gang = 0;  // > 0: items processed by gangs of this many workgroups (gang_kernel)
 };
 // default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu")
 std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
@@ -104,6 +105,8 @@ std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
 struct Group {
   int cls = 0;
   uint32_t first = 0, count = 0;
+  GangParams gp{};     // gang classes: barrier records, scratch, error word
+  int gang_grid = 0;   // gang classes: workgroups launched (gangs x members)
 };
 
 struct Ticket {
@@ -117,6 +120,8 @@ struct Ticket {
   std::vector<ItemDesc> items;
   std::vector<uint8_t> h_rec;
   std::vector<uint8_t> h_meta;                      // staging kept alive until the ticket finishes
+  Alloc gang;                                       // gang barrier records and scratch
+  std::vector<int> h_gang_err;                      // one error word per gang group
   std::vector<std::vector<uint8_t>> h_inits;
   unsigned long long* d_counters = nullptr;
   unsigned long long h_counters[2 + SDIM_NUM_TIMERS] = {0};
@@ -150,6 +155,11 @@ struct sdim_ctx {
   int64_t allow_lds = 1;
   int64_t hbm_threads = 512;
   int64_t hbm_per_cu = 2;
+  int64_t gang_min_n = 1536;  // circuits with at least this many qudits run on gangs (0: never)
+  int64_t gang_chunk = 64;    // minimum bytes of each member's share of a row
+  int64_t gang_threads = 512;
+  int64_t gang_max = 0;       // most members per gang (0: every CU)
+  hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
 };
 
 namespace {
@@ -414,12 +424,31 @@ int finish(sdim_ctx* c, Ticket* t) {
   if (t->finished) return 0;
   HIP_OK(hipStreamSynchronize(t->stream));
   t->finished = true;
+  for (int e : t->h_gang_err)
+    if (e) return fail(SDIM_EHIP, "a gang barrier timed out (a member was not resident); results are invalid");
   return 0;
 }
 
+// frees a ticket's device memory and graphs and forgets it
+void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::iterator it) {
+  Ticket* t = it->second.get();
+  ctx_free(c, t->arena);
+  ctx_free(c, t->meta);
+  ctx_free(c, t->gang);
+  for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
+  if (t->ev0) (void)hipEventDestroy(t->ev0);
+  if (t->ev1) (void)hipEventDestroy(t->ev1);
+  c->tickets.erase(it);
+}
+
+
 // Enqueue the device work of a ticket on its stream.  which: 0 all items,
 // 1 HBM-resident items only, 2 LDS-resident items only (batched mode).
-int enqueue(Ticket* T, int which) {
+// gang_gate: when set, every gang launch waits for the previous one (on any stream)
+// and records its own completion there, so two gang kernels never share the device:
+// a gang waits for all of its members, and two partially resident gangs could
+// otherwise hold the CUs the other one needs.
+int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
   KParams& P = T->P;
   if (T->mode == SDIM_MODE_BATCHED) {
     if (launch_reset(T->d_queue, P.counters, T->stream)) return fail(SDIM_EHIP, "reset launch failed");
@@ -432,6 +461,13 @@ int enqueue(Ticket* T, int which) {
       Q.n_items = g.count;
       Q.queue = T->d_queue + g.cls;
       Q.lds_bytes = std::min<uint32_t>(L.lds, P.lds_bytes);
+      if (L.gang) {
+        if (gang_gate && hipStreamWaitEvent(T->stream, gang_gate, 0) != hipSuccess)
+          return fail(SDIM_EHIP, "hipStreamWaitEvent failed");
+        if (launch_gang(Q, g.gp, L.nt, g.gang_grid, T->stream)) return fail(SDIM_EHIP, "gang kernel launch failed");
+        if (gang_gate && hipEventRecord(gang_gate, T->stream) != hipSuccess) return fail(SDIM_EHIP, "hipEventRecord failed");
+        continue;
+      }
       const int nt = T->threads ? T->threads : L.nt;
       const int per_cu = std::max(1, std::min(L.per_cu, 1024 / nt));
       const int grid = T->grid_override ? T->grid_override
@@ -488,6 +524,7 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
     c->streams.push_back(st);
   }
   c->stream = c->streams[0];
+  HIP_OK(hipEventCreateWithFlags(&c->gang_done, hipEventDisableTiming));
   if (prepare_kernels()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
   *out = c.release();
   return 0;
@@ -502,11 +539,14 @@ int sdim_ctx_destroy(sdim_ctx* c) {
     (void)hipStreamSynchronize(t->stream);
     if (t->arena.p) (void)hipFree(t->arena.p);
     if (t->meta.p) (void)hipFree(t->meta.p);
+    if (t->gang.p) (void)hipFree(t->gang.p);
+    for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
   }
   for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
   for (auto st : c->streams) (void)hipStreamDestroy(st);
+  if (c->gang_done) (void)hipEventDestroy(c->gang_done);
   delete c;
   return 0;
 }
@@ -535,6 +575,18 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "hbm_per_cu") {
     if (v < 1 || v > 16) return fail(SDIM_EINVAL, "hbm_per_cu must be 1..16");
     c->hbm_per_cu = v;
+  } else if (k == "gang_min_n") {
+    if (v < 0) return fail(SDIM_EINVAL, "gang_min_n must be >= 0");
+    c->gang_min_n = v;
+  } else if (k == "gang_chunk") {
+    if (v < 16 || v > 65536 || (v & 15)) return fail(SDIM_EINVAL, "gang_chunk must be a multiple of 16 in [16, 65536]");
+    c->gang_chunk = v;
+  } else if (k == "gang_threads") {
+    if (v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "gang_threads must be 256, 512 or 1024");
+    c->gang_threads = v;
+  } else if (k == "gang_max") {
+    if (v < 0) return fail(SDIM_EINVAL, "gang_max must be >= 0");
+    c->gang_max = v;
   } else {
     return fail(SDIM_EINVAL, "unknown option " + k);
   }
@@ -553,6 +605,10 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "hbm_threads") *v = c->hbm_threads;
   else if (k == "hbm_per_cu") *v = c->hbm_per_cu;
   else if (k == "num_cu") *v = c->n_cu;
+  else if (k == "gang_min_n") *v = c->gang_min_n;
+  else if (k == "gang_chunk") *v = c->gang_chunk;
+  else if (k == "gang_threads") *v = c->gang_threads;
+  else if (k == "gang_max") *v = c->gang_max;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -610,7 +666,39 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     const bool has_noise = keep.size() > keep0;
     uint32_t S = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : auto_S);
     int cls = -1;
-    if (mode == SDIM_MODE_BATCHED) {
+    uint32_t gang_cb = 0;     // gang items: LDS width of a member's columns
+    if (mode == SDIM_MODE_BATCHED && c->gang_min_n > 0 && j.n >= c->gang_min_n) {
+      // a gang of K workgroups, each owning >= gang_chunk bytes of every row; gang
+      // workgroups take the whole LDS of a CU, so one is resident per CU
+      const int gnt = (int)c->gang_threads;
+      uint64_t kmax = (uint64_t)c->n_cu;
+      if (c->gang_max) kmax = std::min<uint64_t>(kmax, (uint64_t)c->gang_max);
+      uint32_t K = 1;
+      while ((uint64_t)K * 2 <= kmax && (uint64_t)K * 2 * (uint64_t)c->gang_chunk <= rs) K *= 2;
+      if (K >= 2) {
+        const uint32_t bpw = d == 2 ? 8 : 4;
+        const uint32_t cb = gang_words(rs / bpw, K, bpw) * bpw;
+        const uint32_t lds = std::min<uint32_t>(163584, cap);
+        uint32_t Sg = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
+        auto gneed = [&](uint32_t s) -> uint64_t {
+          return (uint64_t)scratch_bytes(n, cb, s, (uint32_t)gnt) + (has_noise ? noise_bytes(1, cb, s) : 0);
+        };
+        while (Sg > 1 && gneed(Sg) > lds) Sg /= 2;
+        if (gneed(Sg) <= lds) {
+          S = Sg;
+          gang_cb = cb;
+          for (int k = 0; k < (int)T->classes.size() && cls < 0; k++)
+            if (T->classes[k].gang == K && T->classes[k].nt == gnt) cls = k;
+          if (cls < 0) {
+            if (T->classes.size() >= SDIM_MAX_CLASSES) return fail(SDIM_EINVAL, "too many launch classes");
+            LaunchClass L{false, gnt, lds, 1, K};
+            T->classes.push_back(L);
+            cls = (int)T->classes.size() - 1;
+          }
+        }
+      }
+    }
+    if (mode == SDIM_MODE_BATCHED && cls < 0) {
       auto need = [&](int k, uint32_t s) -> uint64_t {
         const LaunchClass& L = kClasses[k];
         return (L.resident ? ((tableau_bytes(n, d, s) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, s, nt_of(c, L)) +
@@ -638,12 +726,14 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     // noise ops per segment that fit next to the tableau / scratch
     if (mode == SDIM_MODE_BATCHED) {
       const LaunchClass& L = kClasses[cls];
-      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, S, nt_of(c, L));
+      const uint32_t w = gang_cb ? gang_cb : rs;     // width of the per-column LDS arrays
+      const uint32_t nt = gang_cb ? (uint32_t)L.nt : nt_of(c, L);
+      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, w, S, nt);
       const uint64_t budget = std::min<uint32_t>(L.lds, cap);
-      // HBM-resident items keep half of the remaining LDS for the column tiles of gate segments
-      const uint64_t room = budget > base ? (resident ? budget - base : (budget - base) / 2) : 0;
+      // HBM-resident one-workgroup items keep half of the remaining LDS for the column tiles of gate segments
+      const uint64_t room = budget > base ? ((resident || gang_cb) ? budget - base : (budget - base) / 2) : 0;
       uint32_t nn_max = 0;
-      while (nn_max < 65535 && noise_bytes(nn_max + 1, rs, S) <= room) nn_max++;
+      while (nn_max < 65535 && noise_bytes(nn_max + 1, w, S) <= room) nn_max++;
       if (has_noise && nn_max == 0) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
       jp.ops_off = (uint32_t)ops.size();
       build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, ops);
@@ -708,17 +798,52 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       i = e;
     }
   }
+  // gang groups: as many gangs as fit on the device at once (and no more than items)
+  uint64_t gang_bytes = 0;
+  for (Group& g : T->groups) {
+    const LaunchClass& L = T->classes[g.cls];
+    if (!L.gang) continue;
+    uint32_t n_max = 1, S_max = 1;
+    for (uint32_t i = g.first; i < g.first + g.count; i++) {
+      n_max = std::max(n_max, items[order[i]].n);
+      S_max = std::max(S_max, items[order[i]].S);
+    }
+    const uint32_t n_gangs = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->n_cu / L.gang, g.count));
+    g.gang_grid = (int)(n_gangs * L.gang);
+    g.gp.K = L.gang;
+    g.gp.n_max = n_max;
+    g.gp.S_max = S_max;
+    g.gp.scr_stride = gang_scratch_bytes(L.gang, n_max, S_max);
+    // layout inside the ticket's gang buffer: [error word][sync records][scratch]
+    g.gp.err = (int*)(uintptr_t)gang_bytes;
+    gang_bytes += 256;
+    g.gp.sync = (GangSync*)(uintptr_t)gang_bytes;
+    gang_bytes += 256ull * n_gangs;
+    g.gp.scr = (uint8_t*)(uintptr_t)gang_bytes;
+    gang_bytes += g.gp.scr_stride * n_gangs;
+  }
   const uint64_t o_items = 0;
   const uint64_t o_order = align256(o_items + items.size() * sizeof(ItemDesc));
   const uint64_t o_ops = align256(o_order + order.size() * 4);
   const uint64_t o_keep = align256(o_ops + ops.size() * sizeof(DevOp));
   const uint64_t o_queue = align256(o_keep + keep.size() * 8);
-  const uint64_t o_cnt = align256(o_queue + 32);
+  const uint64_t o_cnt = align256(o_queue + 4 * SDIM_MAX_CLASSES);
   const uint64_t meta_bytes = align256(o_cnt + 8 * (2 + SDIM_NUM_TIMERS));
   int rc = ctx_alloc(c, arena_bytes, T->arena);
   if (rc) return rc;
   rc = ctx_alloc(c, meta_bytes, T->meta);
   if (rc) { ctx_free(c, T->arena); return rc; }
+  if (gang_bytes) {
+    rc = ctx_alloc(c, gang_bytes, T->gang);
+    if (rc) { ctx_free(c, T->arena); ctx_free(c, T->meta); return rc; }
+    uint8_t* gb = (uint8_t*)T->gang.p;
+    for (Group& g : T->groups) {
+      if (!T->classes[g.cls].gang) continue;
+      g.gp.err = (int*)(gb + (uintptr_t)g.gp.err);
+      g.gp.sync = (GangSync*)(gb + (uintptr_t)g.gp.sync);
+      g.gp.scr = gb + (uintptr_t)g.gp.scr;
+    }
+  }
   uint8_t* meta = (uint8_t*)T->meta.p;
   uint8_t* arena = (uint8_t*)T->arena.p;
   std::vector<uint8_t>& h_meta = T->h_meta;
@@ -755,17 +880,21 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   T->threads = (int)c->threads;
 
   HIP_OK(hipEventRecord(T->ev0, T->stream));
-  rc = enqueue(T.get(), 0);
+  rc = enqueue(T.get(), 0, c->gang_done);
   if (rc) return rc;
   HIP_OK(hipEventRecord(T->ev1, T->stream));
   // records back to the host (async; valid after wait)
   T->h_rec.resize(T->rec_bytes);
   if (T->rec_bytes) HIP_OK(hipMemcpyAsync(T->h_rec.data(), arena, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
   HIP_OK(hipMemcpyAsync(T->h_counters, T->d_counters, sizeof(T->h_counters), hipMemcpyDeviceToHost, T->stream));
+  T->h_gang_err.assign(T->groups.size(), 0);
+  for (size_t gi = 0; gi < T->groups.size(); gi++)
+    if (T->classes[T->groups[gi].cls].gang)
+      HIP_OK(hipMemcpyAsync(&T->h_gang_err[gi], T->groups[gi].gp.err, sizeof(int), hipMemcpyDeviceToHost, T->stream));
 
   T->id = c->next_ticket++;
   *ticket = T->id;
-  if (device_bytes) *device_bytes = T->arena.bytes + T->meta.bytes;
+  if (device_bytes) *device_bytes = T->arena.bytes + T->meta.bytes + T->gang.bytes;
   c->tickets[T->id] = std::move(T);
   return 0;
 }
@@ -804,14 +933,8 @@ int sdim_release(sdim_ctx* c, uint64_t ticket) {
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->tickets.find(ticket);
   if (it == c->tickets.end()) return fail(SDIM_ENOTFOUND, "unknown ticket");
-  Ticket* t = it->second.get();
-  int rc = finish(c, t);
-  ctx_free(c, t->arena);
-  ctx_free(c, t->meta);
-  for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
-  if (t->ev0) (void)hipEventDestroy(t->ev0);
-  if (t->ev1) (void)hipEventDestroy(t->ev1);
-  c->tickets.erase(it);
+  int rc = finish(c, it->second.get());
+  drop_ticket(c, it);
   return rc;
 }
 
@@ -897,6 +1020,29 @@ int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t*
   return 0;
 }
 
+int sdim_ticket_info(sdim_ctx* c, uint64_t ticket, const char* key, int64_t* value) {
+  if (!c || !key || !value) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  const std::string k(key);
+  if (k == "items") { *value = (int64_t)t->items.size(); return 0; }
+  if (k == "lds_items") { *value = t->n_lds_items; return 0; }
+  if (k == "gang_items" || k == "gang_members" || k == "gangs" || k == "launches") {
+    int64_t items = 0, members = 0, gangs = 0;
+    for (const Group& g : t->groups) {
+      const LaunchClass& L = t->classes[g.cls];
+      if (!L.gang) continue;
+      items += g.count;
+      members = std::max<int64_t>(members, L.gang);
+      gangs += g.gang_grid / (int)L.gang;
+    }
+    *value = k == "gang_items" ? items : k == "gang_members" ? members : k == "gangs" ? gangs : (int64_t)t->groups.size();
+    return 0;
+  }
+  return fail(SDIM_EINVAL, "unknown ticket key " + k);
+}
+
 int sdim_ticket_counters(sdim_ctx* c, uint64_t ticket, uint64_t* out, int32_t count) {
   if (!c || !out || count < 0) return fail(SDIM_EINVAL, "bad arguments");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -944,6 +1090,12 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
   if (t->rec_bytes)
     HIP_OK(hipMemcpyAsync(t->h_rec.data(), t->arena.p, t->rec_bytes, hipMemcpyDeviceToHost, t->stream));
   HIP_OK(hipStreamSynchronize(t->stream));
+  for (size_t gi = 0; gi < t->groups.size(); gi++) {
+    if (!t->classes[t->groups[gi].cls].gang) continue;
+    int e = 0;
+    HIP_OK(hipMemcpy(&e, t->groups[gi].gp.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) return fail(SDIM_EHIP, "a gang barrier timed out during replay; results are invalid");
+  }
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
   if (elapsed_ms) *elapsed_ms = ms;
@@ -1072,14 +1224,7 @@ void serve_loop(sdim_server* s) {
           out.emplace_back(b.reqs[k], std::move(buf));
         }
         auto it = c->tickets.find(b.ticket);
-        if (it != c->tickets.end()) {
-          ctx_free(c, it->second->arena);
-          ctx_free(c, it->second->meta);
-          for (auto& g : it->second->graph) if (g) (void)hipGraphExecDestroy(g);
-          if (it->second->ev0) (void)hipEventDestroy(it->second->ev0);
-          if (it->second->ev1) (void)hipEventDestroy(it->second->ev1);
-          c->tickets.erase(it);
-        }
+        if (it != c->tickets.end()) drop_ticket(c, it);
       }
       {
         std::lock_guard<std::mutex> lk(s->mu);

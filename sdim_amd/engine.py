@@ -116,6 +116,12 @@ class Ticket:
         return {"kernel_ms": ms.value, "bytes_hbm": bh.value, "bytes_lds": bl.value,
                 "items": ni.value, "lds_items": nl.value}
 
+    def info(self, key):
+        """Device layout of the ticket: "items", "lds_items", "gang_items", "gang_members", "gangs", "launches"."""
+        v = ctypes.c_int64(0)
+        _lib.check(self.engine.L.sdim_ticket_info(self.engine.ctx, self.id, key.encode(), ctypes.byref(v)))
+        return v.value
+
     def counters(self):
         """Raw device counters: bytes (HBM-resident, LDS-resident) and phase timer ticks."""
         buf = (ctypes.c_uint64 * 14)()
