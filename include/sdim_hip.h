@@ -85,7 +85,8 @@ const char* sdim_last_error(void);
  *   "hbm_per_cu"      workgroups per CU of that class (its LDS budget is 160 KB / hbm_per_cu)
  *   "gang_min_n"      circuits with at least this many qudits run on a gang of workgroups,
  *                     each owning a range of generator columns (0 = never; default 1536)
- *   "gang_chunk"      minimum bytes of a row each gang member owns (multiple of 16; default 64)
+ *   "gang_chunk"      minimum bytes of a row each gang member owns (multiple of 16; default 128,
+ *                     one cache line, so that no two members write the same line)
  *   "gang_threads"    workgroup size of the gang kernel: 256, 512 (default) or 1024
  *   "gang_max"        most workgroups per gang (0 = one per CU)
  */

@@ -7,7 +7,7 @@
 namespace sdim {
 
 // number of optional phase timers after the two byte counters
-#define SDIM_NUM_TIMERS 12
+#define SDIM_NUM_TIMERS 16
 // launch classes (one item queue each)
 #define SDIM_MAX_CLASSES 32
 
@@ -49,7 +49,8 @@ __host__ __device__ inline uint32_t noise_bytes(uint32_t nn, uint32_t rs, uint32
 struct GangSync;
 struct GangParams {
   GangSync* sync;       // one 256-byte record per gang
-  uint8_t* scr;         // per gang: [n_max][Kp] + [S_max][Kp] bytes, then acc[K], crs[K], tot[S_max] u32
+  uint8_t* scr;         // per gang: [n_max][Kp] + [S_max][Kp] bytes, acc[K], crs[K], tot[S_max] u32,
+                        // two pivot-column staging buffers
   uint64_t scr_stride;  // bytes of one gang's scratch
   uint32_t K;           // members per gang
   uint32_t n_max, S_max;
@@ -65,13 +66,18 @@ __host__ __device__ inline uint32_t gang_words(uint32_t words, uint32_t K, uint3
   uint32_t c = (words + K - 1) / K;
   return (c + unit - 1) / unit * unit;
 }
+// staging of a pivot column: n bytes (x | z << 4 per row) then S pivot phases, two buffers
+__host__ __device__ inline uint64_t gang_stage_bytes(uint32_t n_max, uint32_t S_max) {
+  return gang_align((uint64_t)n_max + S_max);
+}
 __host__ __device__ inline uint64_t gang_scratch_bytes(uint32_t K, uint32_t n_max, uint32_t S_max) {
   const uint64_t kp = gang_kp(K);
-  return gang_align(n_max * kp) + gang_align(S_max * kp) + gang_align(4ull * (2 * K + S_max));
+  return gang_align(n_max * kp) + gang_align(S_max * kp) + gang_align(4ull * (2 * K + S_max)) +
+         2 * gang_stage_bytes(n_max, S_max);
 }
 
 int prepare_kernels();
-int launch_gang(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st);
+int launch_gang(const KParams& P, const GangParams& G, int d, int nt, int grid, hipStream_t st);
 int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st);
 int launch_plain_op(const KParams& P, const DevOp& o, uint32_t n, uint32_t d, uint32_t n_items, uint32_t S,

@@ -1039,6 +1039,7 @@ struct GangCtx {
   uint32_t* acc;       // [K] local scan sums
   uint32_t* crs;       // [K] cross-term sums
   uint32_t* tot;       // [S] per-shot totals
+  uint8_t* stage[2];   // pivot column staging (by measurement parity): x | z << 4 per row, then pivot phases
   uint32_t K, Kp, me;
   int* gerr;
   uint32_t target;     // arrivals expected at the next barrier
@@ -1049,8 +1050,9 @@ SDIM_DEV bool gang_barrier(GangCtx& g, int* s_ok) {
   g.target += g.K;
   if (threadIdx.x == 0) {
     int ok = 1;
-    __threadfence();
-    __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // release: this workgroup's writes (ordered before thread 0 by the barrier above) become
+    // visible device-wide before the arrival is counted
+    __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t t0 = wall_clock64();
     while (__hip_atomic_load(&g.sy->ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g.target) {
       if (__hip_atomic_load(&g.sy->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
@@ -1062,7 +1064,7 @@ SDIM_DEV bool gang_barrier(GangCtx& g, int* s_ok) {
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // later reads see the other members' writes
     *s_ok = ok;
   }
   __syncthreads();
@@ -1136,7 +1138,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
                            GangCtx& Gc, int* s_ok, unsigned long long& bytes) {
   constexpr bool BIN = (D == 2);
   constexpr uint32_t CPW = BIN ? 64 : 4;
-  constexpr int RB = 8;     // rows per load batch in the sweep
+  constexpr int RB = 8;     // rows per load batch in the sweep (16 bytes of X and Z each)
   const uint32_t tid = threadIdx.x;
   const uint32_t n = Tg.n, rs = Tg.rs, Sh = Tg.S;
   const uint32_t nw = V.nw;
@@ -1147,40 +1149,57 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
   uint8_t* rec = P.arena + it.rec_off + (size_t)slot * Sh;
   const bool lead = Gc.me == 0;
 
-  // ---- A: own words of row q -> pivot candidate, f, words with f != 0
-  for (uint32_t w = tid; w < nw; w += NT) {
-    S.DOT[2 * w] = 0;
-    S.DOT[2 * w + 1] = 0;
-    const uint32_t gw = w0 + w;
+  // phase timers (gang): 0 A, 1 barrier 1, 2 column pack, 3 barrier 2, 4 unpack, 5 sweep, 6 D, 10 E, 12 det
+  SDIM_TMARK(tA);
+  // ---- A: own 16-byte groups of row q -> pivot candidate, f, groups with f != 0
+  constexpr uint32_t WPG = BIN ? 2 : 4;              // words per 16-byte group
+  const uint32_t ngr = nw / WPG;
+  for (uint32_t g = tid; g < ngr; g += NT) {
+    const uint4 v4 = *(const uint4*)(V.X + (size_t)q * rs + 16 * g);
+    uint4 f4;
+    bool any;
     if constexpr (BIN) {
-      const uint64_t v = row64(V.X, q, rs)[w];
-      ((uint64_t*)S.F)[w] = v;
-      if (v) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
-      if (gw < nsw) {
-        uint64_t vs = v;
-        const uint32_t lim = n - 64 * gw;
-        if (lim < 64) vs &= (1ull << lim) - 1;
-        if (vs) atomicMin(&M[0], (int)(64 * gw + __builtin_ctzll(vs)));
+      const uint64_t v0 = (uint64_t)v4.x | ((uint64_t)v4.y << 32), v1 = (uint64_t)v4.z | ((uint64_t)v4.w << 32);
+      f4 = v4;                                        // f = -x = x mod 2
+      any = (v0 | v1) != 0;
+      #pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const uint32_t gw = w0 + 2 * g + k;
+        uint64_t vs = k ? v1 : v0;
+        if (gw < nsw && vs) {
+          const uint32_t lim = n - 64 * gw;
+          if (lim < 64) vs &= (1ull << lim) - 1;
+          if (vs) atomicMin(&M[0], (int)(64 * gw + __builtin_ctzll(vs)));
+        }
       }
-      bytes += 8;
     } else {
-      const uint32_t v = row32(V.X, q, rs)[w];
-      const uint32_t f = negm<D>(v);
-      ((uint32_t*)S.F)[w] = f;
-      if (f) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
-      if (gw < nsw) {
-        uint32_t vs = v;
-        const uint32_t lim = n - 4 * gw;
-        if (lim < 4) vs &= (1u << (8 * lim)) - 1;
-        if (vs) atomicMin(&M[0], (int)(4 * gw + (__builtin_ctz(vs) >> 3)));
+      const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+      f4 = make_uint4(negm<D>(v4.x), negm<D>(v4.y), negm<D>(v4.z), negm<D>(v4.w));
+      any = (f4.x | f4.y | f4.z | f4.w) != 0;
+      #pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t gw = w0 + 4 * g + k;
+        uint32_t vs = vv[k];
+        if (gw < nsw && vs) {
+          const uint32_t lim = n - 4 * gw;
+          if (lim < 4) vs &= (1u << (8 * lim)) - 1;
+          if (vs) atomicMin(&M[0], (int)(4 * gw + (__builtin_ctz(vs) >> 3)));
+        }
       }
-      bytes += 4;
     }
+    *(uint4*)(S.F + 16 * g) = f4;
+    *(uint4*)(S.DOT + 2 * WPG * g) = make_uint4(0, 0, 0, 0);                // two dot words per column word
+    if constexpr (!BIN) *(uint4*)(S.DOT + 2 * WPG * g + 4) = make_uint4(0, 0, 0, 0);
+    if (any) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)g;
+    bytes += 16;
   }
   __syncthreads();
   if (tid == 0 && M[0] != INT_MAX)
     __hip_atomic_fetch_min(&Gc.sy->piv[mk & 1], M[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  SDIM_TADD(P, 0, tA);
+  SDIM_TMARK(tG1);
   if (!gang_barrier(Gc, s_ok)) return false;                                // G1
+  SDIM_TADD(P, 1, tG1);
   const int piv = __hip_atomic_load(&Gc.sy->piv[mk & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (tid == 0) {
     Mn[0] = INT_MAX; Mn[1] = 0; Mn[2] = 0; Mn[3] = 0;
@@ -1192,24 +1211,34 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     const uint32_t p = (uint32_t)piv;
     const bool own_p = p >= c_lo && p < c_hi;
     const bool own_g = n + p >= c_lo && n + p < c_hi;
-    // ---- B: every member gathers column p (whole tableau), touched rows, <x_p, z_p>, pivot phases
-    for (uint32_t r = tid; r < n; r += NT) {
-      uint32_t xr, zr;
-      if constexpr (BIN) {
-        xr = (uint32_t)((row64(Tg.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-        zr = (uint32_t)((row64(Tg.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
-        if (own_p) bytes += 16;
-      } else {
-        xr = Tg.X[(size_t)r * rs + p];
-        zr = Tg.Z[(size_t)r * rs + p];
-        if (own_p) bytes += 2;
+    uint8_t* stg = Gc.stage[mk & 1];
+    SDIM_TMARK(tB);
+    // ---- B: every member packs its slice of rows of column p (x | z << 4 per row) and of the
+    // pivot phases into the staging buffer: one round of strided loads per member
+    {
+      const uint32_t r0 = (uint32_t)(((uint64_t)n * Gc.me) / Gc.K), r1 = (uint32_t)(((uint64_t)n * (Gc.me + 1)) / Gc.K);
+      for (uint32_t r = r0 + tid; r < r1; r += NT) {
+        uint32_t xr, zr;
+        if constexpr (BIN) {
+          xr = (uint32_t)((row64(Tg.X, r, rs)[p >> 6] >> (p & 63)) & 1);
+          zr = (uint32_t)((row64(Tg.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+        } else {
+          xr = Tg.X[(size_t)r * rs + p];
+          zr = Tg.Z[(size_t)r * rs + p];
+        }
+        stg[r] = (uint8_t)(xr | (zr << 4));
       }
-      S.xp[r] = (uint8_t)xr;
-      S.zp[r] = (uint8_t)zr;
-      if (xr | zr) {
-        S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
-        if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
+      const uint32_t s0 = (uint32_t)(((uint64_t)Sh * Gc.me) / Gc.K), s1 = (uint32_t)(((uint64_t)Sh * (Gc.me + 1)) / Gc.K);
+      for (uint32_t s = s0 + tid; s < s1; s += NT) {
+        if constexpr (BIN) {
+          const uint32_t v0 = (uint32_t)((plane(Tg, s, 0)[p >> 6] >> (p & 63)) & 1);
+          const uint32_t v1 = (uint32_t)((plane(Tg, s, 1)[p >> 6] >> (p & 63)) & 1);
+          stg[n + s] = (uint8_t)(v0 | (v1 << 1));
+        } else {
+          stg[n + s] = phase_vec(Tg, s)[p];
+        }
       }
+      if (own_p) bytes += BIN ? 16ull * n : 2ull * n;
     }
     if (tid == 0) {
       if constexpr (BIN) {
@@ -1220,17 +1249,63 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         if (own_g) S.F[n + p - c_lo] = 0;
       }
     }
-    for (uint32_t s = tid; s < Sh; s += NT) {
-      if constexpr (BIN) {
-        const uint32_t v0 = (uint32_t)((plane(Tg, s, 0)[p >> 6] >> (p & 63)) & 1);
-        const uint32_t v1 = (uint32_t)((plane(Tg, s, 1)[p >> 6] >> (p & 63)) & 1);
-        S.shot[s] = v0 | (v1 << 1);
-      } else {
-        S.shot[s] = phase_vec(Tg, s)[p];
+    SDIM_TADD(P, 2, tB);
+    SDIM_TMARK(tG2);
+    if (!gang_barrier(Gc, s_ok)) return false;                              // G2: column p staged
+    SDIM_TADD(P, 3, tG2);
+    SDIM_TMARK(tU);
+    // every member: unpack the pivot column, touched rows (compacted per wave), <x_p, z_p>,
+    // pivot phases.  Four rows per 32-bit word, UW words per thread in flight.
+    {
+      constexpr uint32_t UW = 4;
+      const uint32_t nwr = (n + 3) / 4;                 // staging words holding rows
+      const uint32_t lane = tid & 63;
+      uint32_t xzs = 0;
+      for (uint32_t j0 = 0; j0 < nwr; j0 += NT * UW) {
+        uint32_t wv[UW];
+        #pragma unroll
+        for (uint32_t u = 0; u < UW; u++) {
+          const uint32_t j = j0 + u * NT + tid;
+          wv[u] = j < nwr ? ((const uint32_t*)stg)[j] : 0u;
+        }
+        #pragma unroll
+        for (uint32_t u = 0; u < UW; u++) {
+          const uint32_t j = j0 + u * NT + tid;
+          uint32_t v = wv[u];
+          if (4 * j + 4 > n && j < nwr) v &= (1u << (8 * (n - 4 * j))) - 1;   // bytes past row n-1
+          uint32_t cnt = 0;
+          #pragma unroll
+          for (int b = 0; b < 4; b++) cnt += ((v >> (8 * b)) & 0xFFu) ? 1u : 0u;
+          // wave-exclusive prefix of the counts, one LDS atomic per wave
+          uint32_t x = cnt;
+          #pragma unroll
+          for (int o = 1; o < 64; o <<= 1) { const uint32_t y = __shfl_up(x, o, 64); if (lane >= (uint32_t)o) x += y; }
+          const uint32_t tot = __shfl(x, 63, 64);
+          uint32_t base = 0;
+          if (lane == 0 && tot) base = (uint32_t)atomicAdd(&M[1], (int)tot);
+          base = __shfl(base, 0, 64) + x - cnt;
+          if (j < nwr) {
+            #pragma unroll
+            for (int b = 0; b < 4; b++) {
+              const uint32_t r = 4 * j + b;
+              if (r < n) {
+                const uint32_t e8 = (v >> (8 * b)) & 0xFFu;
+                const uint32_t xr = e8 & 15u, zr = e8 >> 4;
+                S.xp[r] = (uint8_t)xr;
+                S.zp[r] = (uint8_t)zr;
+                if (e8) S.TR[base++] = (uint16_t)r;
+                xzs += xr * zr;
+              }
+            }
+          }
+        }
       }
+      if constexpr (!BIN) if (xzs) atomicAdd(&M[3], (int)(xzs % D));
     }
-    // every member holds column p before its owner rewrites it
-    if (!gang_barrier(Gc, s_ok)) return false;                              // G2
+    for (uint32_t s = tid; s < Sh; s += NT) S.shot[s] = stg[n + s];
+    __syncthreads();
+    SDIM_TADD(P, 4, tU);
+    SDIM_TMARK(tC);
     const uint32_t nT = (uint32_t)M[1], nF = (uint32_t)M[2];
     uint32_t e = 1, xz = 0, pinc = 0;
     if constexpr (!BIN) {
@@ -1242,49 +1317,63 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
     }
 
-    // ---- C: sweep (touched rows) x (own words with f != 0)
+    // ---- C: sweep (touched rows) x (own 16-byte groups with f != 0)
     if (nT && nF) {
       uint32_t NRG, j0, g0, jstep;
       if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
       else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
       if (g0 < NRG) {
         for (uint32_t j = j0; j < nF; j += jstep) {
-          const uint32_t w = S.FL[j];
+          const uint32_t gi = S.FL[j];
+          const uint4 f4 = *(const uint4*)(S.F + 16 * gi);
           if constexpr (BIN) {
-            const uint64_t f = ((const uint64_t*)S.F)[w];
-            uint64_t dacc = 0;
+            const uint64_t f0 = (uint64_t)f4.x | ((uint64_t)f4.y << 32), f1 = (uint64_t)f4.z | ((uint64_t)f4.w << 32);
+            uint64_t da0 = 0, da1 = 0;
             for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
               uint32_t rr[RB], sxv[RB], szv[RB];
-              uint64_t xv[RB], zv[RB];
+              uint4 xv[RB], zv[RB];
               #pragma unroll
               for (int b = 0; b < RB; b++) {
                 const uint32_t kk = k0 + (uint32_t)b * NRG;
                 if (kk < nT) {
                   rr[b] = S.TR[kk];
                   sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
-                  zv[b] = row64(V.Z, rr[b], rs)[w];
-                  xv[b] = sxv[b] ? row64(V.X, rr[b], rs)[w] : 0ull;
+                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
+                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
                 }
               }
               #pragma unroll
               for (int b = 0; b < RB; b++) {
                 const uint32_t kk = k0 + (uint32_t)b * NRG;
                 if (kk < nT) {
-                  if (sxv[b]) { dacc ^= zv[b]; row64(V.X, rr[b], rs)[w] = xv[b] ^ f; bytes += 16; }
-                  if (szv[b]) { row64(V.Z, rr[b], rs)[w] = zv[b] ^ f; bytes += 8; }
-                  bytes += 8;
+                  const uint64_t z0 = (uint64_t)zv[b].x | ((uint64_t)zv[b].y << 32), z1 = (uint64_t)zv[b].z | ((uint64_t)zv[b].w << 32);
+                  if (sxv[b]) {
+                    da0 ^= z0; da1 ^= z1;
+                    uint4 o = xv[b];
+                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
+                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = o;
+                    bytes += 32;
+                  }
+                  if (szv[b]) {
+                    uint4 o = zv[b];
+                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
+                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) = o;
+                    bytes += 16;
+                  }
+                  bytes += 16;
                 }
               }
             }
-            if (dacc) {
-              atomicXor(&S.DOT[2 * w], (uint32_t)dacc);
-              atomicXor(&S.DOT[2 * w + 1], (uint32_t)(dacc >> 32));
-            }
+            da0 &= f0; da1 &= f1;
+            if (da0) { atomicXor(&S.DOT[4 * gi], (uint32_t)da0); atomicXor(&S.DOT[4 * gi + 1], (uint32_t)(da0 >> 32)); }
+            if (da1) { atomicXor(&S.DOT[4 * gi + 2], (uint32_t)da1); atomicXor(&S.DOT[4 * gi + 3], (uint32_t)(da1 >> 32)); }
           } else {
-            const uint32_t f = ((const uint32_t*)S.F)[w];
-            uint32_t de = 0, dO = 0, cnt = 0;
+            const uint32_t fw[4] = {f4.x, f4.y, f4.z, f4.w};
+            uint32_t de[4] = {0, 0, 0, 0}, dO[4] = {0, 0, 0, 0};
+            uint32_t cnt = 0;
             for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
-              uint32_t rr[RB], sxv[RB], szv[RB], xv[RB], zv[RB];
+              uint32_t rr[RB], sxv[RB], szv[RB];
+              uint4 xv[RB], zv[RB];
               #pragma unroll
               for (int b = 0; b < RB; b++) {
                 const uint32_t kk = k0 + (uint32_t)b * NRG;
@@ -1292,37 +1381,58 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
                   rr[b] = S.TR[kk];
                   sxv[b] = (e * S.xp[rr[b]]) % D;
                   szv[b] = (e * S.zp[rr[b]]) % D;
-                  zv[b] = row32(V.Z, rr[b], rs)[w];
-                  xv[b] = sxv[b] ? row32(V.X, rr[b], rs)[w] : 0u;
+                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
+                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
                 }
               }
               #pragma unroll
               for (int b = 0; b < RB; b++) {
                 const uint32_t kk = k0 + (uint32_t)b * NRG;
                 if (kk < nT) {
+                  const uint32_t zw[4] = {zv[b].x, zv[b].y, zv[b].z, zv[b].w};
                   if (sxv[b]) {
                     const MulTab& t = S.mtab[sxv[b]];
-                    const uint32_t pz = mult<D>(t, zv[b]);
-                    de += pz & 0x00FF00FFu;
-                    dO += (pz >> 8) & 0x00FF00FFu;
-                    row32(V.X, rr[b], rs)[w] = addm<D>(xv[b], mult<D>(t, f));
-                    bytes += 8;
+                    const uint32_t xw[4] = {xv[b].x, xv[b].y, xv[b].z, xv[b].w};
+                    uint32_t ow[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                      const uint32_t pz = mult<D>(t, zw[k]);
+                      de[k] += pz & 0x00FF00FFu;
+                      dO[k] += (pz >> 8) & 0x00FF00FFu;
+                      ow[k] = addm<D>(xw[k], mult<D>(t, fw[k]));
+                    }
+                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+                    bytes += 32;
                   }
-                  if (szv[b]) { row32(V.Z, rr[b], rs)[w] = addm<D>(zv[b], mult<D>(S.mtab[szv[b]], f)); bytes += 4; }
-                  bytes += 4;
+                  if (szv[b]) {
+                    const MulTab& t = S.mtab[szv[b]];
+                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) =
+                        make_uint4(addm<D>(zw[0], mult<D>(t, fw[0])), addm<D>(zw[1], mult<D>(t, fw[1])),
+                                   addm<D>(zw[2], mult<D>(t, fw[2])), addm<D>(zw[3], mult<D>(t, fw[3])));
+                    bytes += 16;
+                  }
+                  bytes += 16;
                 }
               }
-              if (++cnt == 512) { de = fold16<D>(de); dO = fold16<D>(dO); cnt = 0; }
+              if (++cnt == 512) {
+                #pragma unroll
+                for (int k = 0; k < 4; k++) { de[k] = fold16<D>(de[k]); dO[k] = fold16<D>(dO[k]); }
+                cnt = 0;
+              }
             }
-            de = fold16<D>(de);
-            dO = fold16<D>(dO);
-            if (de) atomicAdd(&S.DOT[2 * w], de);
-            if (dO) atomicAdd(&S.DOT[2 * w + 1], dO);
+            #pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const uint32_t a = fold16<D>(de[k]), b = fold16<D>(dO[k]);
+              if (a) atomicAdd(&S.DOT[2 * (4 * gi + k)], a);
+              if (b) atomicAdd(&S.DOT[2 * (4 * gi + k) + 1], b);
+            }
           }
         }
       }
     }
     __syncthreads();
+    SDIM_TADD(P, 5, tC);
+    SDIM_TMARK(tD);
 
     // ---- D: own parts of columns p and n+p; per-shot phases over own words with f != 0
     if (own_p || own_g) {
@@ -1363,12 +1473,13 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         }
       }
     }
-    for (uint32_t i = tid; i < Sh * nF; i += NT) {
-      const uint32_t s = i / nF, j = i - s * nF;
-      const uint32_t w = S.FL[j];
+    for (uint32_t i = tid; i < Sh * nF * WPG; i += NT) {
+      const uint32_t s = i / (nF * WPG), jk = i - s * nF * WPG;
+      const uint32_t w = S.FL[jk / WPG] * WPG + jk % WPG;
       if constexpr (BIN) {
         const uint32_t k2 = S.shot[s];
         const uint64_t f = ((const uint64_t*)S.F)[w];
+        if (!f) continue;
         const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
         uint64_t* p0 = plane(V, s, 0) + w;
         uint64_t* p1 = plane(V, s, 1) + w;
@@ -1381,6 +1492,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         bytes += 32;
       } else {
         const uint32_t f = ((const uint32_t*)S.F)[w];
+        if (!f) continue;
         const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
         uint32_t c = 0;
         #pragma unroll
@@ -1396,6 +1508,8 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       }
     }
     __syncthreads();
+    SDIM_TADD(P, 6, tD);
+    SDIM_TMARK(tE);
     // ---- E: outcomes; ph_{n+p} <- ph_p, ph_p <- -m po on the owners' columns
     for (uint32_t s = tid; s < Sh; s += NT) {
       const U64x4 o = draw(it.seed, it.stream, src, it.shot0 + s);
@@ -1423,7 +1537,9 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       S.shot[Sh + s] = m;
       if (lead) rec[s] = (uint8_t)m;
     }
+    SDIM_TADD(P, 10, tE);
   } else {
+    SDIM_TMARK(tDet);
     // ---- deterministic outcome: prod_i S_i^{f_i}, f_i = dx[q, i] = X[q][n + i]
     const uint32_t ns_own = (w0 < nsw) ? ((nsw - w0) < nw ? (nsw - w0) : nw) : 0u;
     for (uint32_t w = tid; w < ns_own; w += NT) {
@@ -1549,6 +1665,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       S.shot[Sh + s] = m;
       if (lead) rec[s] = (uint8_t)(m | 0x80u);
     }
+    SDIM_TADD(P, 12, tDet);
   }
 
   if (reset) {
@@ -1610,11 +1727,14 @@ SDIM_DEV bool gang_item(const ItemDesc& it, const KParams& P, GangCtx& Gc, uint8
   const DevOp* ops = P.ops + it.ops_off;
   uint32_t k = 0;
   bool ok = true;
+  SDIM_TMARK(tItem);
   while (k < it.n_ops) {
     const DevOp o = ops[k];
     const uint32_t kind = op_kind(o);
     if (kind == OP_SEG) {
+      SDIM_TMARK(tSg);
       k = run_segment<D, NT>(ops, k, V, S, it, P, bytes, lbytes);
+      SDIM_TADD(P, 9, tSg);
     } else if (kind == OP_MEAS || kind == OP_RESET) {
       ok = gang_measure<D, NT>(Tg, V, w0, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, Gc, s_ok, bytes);
       if (!ok) break;
@@ -1623,13 +1743,14 @@ SDIM_DEV bool gang_item(const ItemDesc& it, const KParams& P, GangCtx& Gc, uint8
       break;
     }
   }
+  SDIM_TADD(P, 11, tItem);
   const unsigned long long tot = block_sum64<NT>(bytes + lbytes, S.red);
   if (threadIdx.x == 0) atomicAdd(&P.counters[0], tot);
   return ok;
 }
 
-// Persistent gang kernel: gang g = blockIdx.x / K pulls items from P.queue.
-template <int NT>
+// Persistent gang kernel for dimension D: gang g = blockIdx.x / K pulls items from P.queue.
+template <int D, int NT>
 __global__ __launch_bounds__(NT) void gang_kernel(KParams P, GangParams G) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ int s_ok;
@@ -1644,6 +1765,8 @@ __global__ __launch_bounds__(NT) void gang_kernel(KParams P, GangParams G) {
   Gc.acc = (uint32_t*)(Gc.ps + gang_align(G.S_max * (uint64_t)Gc.Kp));
   Gc.crs = Gc.acc + G.K;
   Gc.tot = Gc.crs + G.K;
+  Gc.stage[0] = (uint8_t*)Gc.acc + gang_align(4ull * (2 * G.K + G.S_max));
+  Gc.stage[1] = Gc.stage[0] + gang_stage_bytes(G.n_max, G.S_max);
   Gc.K = G.K;
   Gc.me = blockIdx.x % G.K;
   Gc.gerr = G.err;
@@ -1659,16 +1782,7 @@ __global__ __launch_bounds__(NT) void gang_kernel(KParams P, GangParams G) {
     round++;
     if (idx >= P.n_items) return;
     const ItemDesc it = P.items[P.order[idx]];
-    bool ok = false;
-    switch (it.d) {
-      case 2: ok = gang_item<2, NT>(it, P, Gc, lds, &s_ok, mk); break;
-      case 3: ok = gang_item<3, NT>(it, P, Gc, lds, &s_ok, mk); break;
-      case 5: ok = gang_item<5, NT>(it, P, Gc, lds, &s_ok, mk); break;
-      case 7: ok = gang_item<7, NT>(it, P, Gc, lds, &s_ok, mk); break;
-      case 11: ok = gang_item<11, NT>(it, P, Gc, lds, &s_ok, mk); break;
-      default: break;
-    }
-    if (!ok) return;
+    if (!gang_item<D, NT>(it, P, Gc, lds, &s_ok, mk)) return;
   }
 }
 
@@ -1797,9 +1911,12 @@ int prepare_kernels() {
   if (hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void*)gang_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void*)gang_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void*)gang_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+#define SDIM_SET_GANG(D) \
+  if (hipFuncSetAttribute((const void*)gang_kernel<D, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1; \
+  if (hipFuncSetAttribute((const void*)gang_kernel<D, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1; \
+  if (hipFuncSetAttribute((const void*)gang_kernel<D, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
+  SDIM_SET_GANG(2) SDIM_SET_GANG(3) SDIM_SET_GANG(5) SDIM_SET_GANG(7) SDIM_SET_GANG(11)
+#undef SDIM_SET_GANG
 #define SDIM_SET_PLAIN(D) \
   if (hipFuncSetAttribute((const void*)plain_meas_kernel<D, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return -1;
   SDIM_SET_PLAIN(2) SDIM_SET_PLAIN(3) SDIM_SET_PLAIN(5) SDIM_SET_PLAIN(7) SDIM_SET_PLAIN(11)
@@ -1825,14 +1942,27 @@ int launch_batch(const KParams& P, int nt, int grid, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_gang(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st) {
+template <int D>
+static void launch_gang_d(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st) {
   const size_t lds = P.lds_bytes;
+  switch (nt) {
+    case 256: hipLaunchKernelGGL((gang_kernel<D, 256>), dim3(grid), dim3(256), lds, st, P, G); break;
+    case 512: hipLaunchKernelGGL((gang_kernel<D, 512>), dim3(grid), dim3(512), lds, st, P, G); break;
+    default: hipLaunchKernelGGL((gang_kernel<D, 1024>), dim3(grid), dim3(1024), lds, st, P, G); break;
+  }
+}
+
+// every item of a gang launch has dimension d
+int launch_gang(const KParams& P, const GangParams& G, int d, int nt, int grid, hipStream_t st) {
+  if (nt != 256 && nt != 512 && nt != 1024) return -2;
   const uint32_t n_gangs = (uint32_t)grid / G.K;
   hipLaunchKernelGGL(gang_reset_kernel, dim3(1), dim3(256), 0, st, G.sync, n_gangs, G.err);
-  switch (nt) {
-    case 256: hipLaunchKernelGGL(gang_kernel<256>, dim3(grid), dim3(256), lds, st, P, G); break;
-    case 512: hipLaunchKernelGGL(gang_kernel<512>, dim3(grid), dim3(512), lds, st, P, G); break;
-    case 1024: hipLaunchKernelGGL(gang_kernel<1024>, dim3(grid), dim3(1024), lds, st, P, G); break;
+  switch (d) {
+    case 2: launch_gang_d<2>(P, G, nt, grid, st); break;
+    case 3: launch_gang_d<3>(P, G, nt, grid, st); break;
+    case 5: launch_gang_d<5>(P, G, nt, grid, st); break;
+    case 7: launch_gang_d<7>(P, G, nt, grid, st); break;
+    case 11: launch_gang_d<11>(P, G, nt, grid, st); break;
     default: return -2;
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;

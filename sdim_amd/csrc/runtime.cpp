@@ -87,6 +87,7 @@ struct LaunchClass {
   uint32_t lds;    // dynamic LDS bytes per workgroup
   int per_cu;      // workgroups resident per CU
   uint32_t gang = 0;  // > 0: items processed by gangs of this many workgroups (gang_kernel)
+  int d = 0;          // gang classes: the dimension of their items
 };
 // default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu")
 std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
@@ -156,7 +157,7 @@ struct sdim_ctx {
   int64_t hbm_threads = 512;
   int64_t hbm_per_cu = 2;
   int64_t gang_min_n = 1536;  // circuits with at least this many qudits run on gangs (0: never)
-  int64_t gang_chunk = 64;    // minimum bytes of each member's share of a row
+  int64_t gang_chunk = 128;   // minimum bytes of each member's share of a row (one cache line)
   int64_t gang_threads = 512;
   int64_t gang_max = 0;       // most members per gang (0: every CU)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
@@ -464,7 +465,7 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
       if (L.gang) {
         if (gang_gate && hipStreamWaitEvent(T->stream, gang_gate, 0) != hipSuccess)
           return fail(SDIM_EHIP, "hipStreamWaitEvent failed");
-        if (launch_gang(Q, g.gp, L.nt, g.gang_grid, T->stream)) return fail(SDIM_EHIP, "gang kernel launch failed");
+        if (launch_gang(Q, g.gp, L.d, L.nt, g.gang_grid, T->stream)) return fail(SDIM_EHIP, "gang kernel launch failed");
         if (gang_gate && hipEventRecord(gang_gate, T->stream) != hipSuccess) return fail(SDIM_EHIP, "hipEventRecord failed");
         continue;
       }
@@ -688,10 +689,10 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
           S = Sg;
           gang_cb = cb;
           for (int k = 0; k < (int)T->classes.size() && cls < 0; k++)
-            if (T->classes[k].gang == K && T->classes[k].nt == gnt) cls = k;
+            if (T->classes[k].gang == K && T->classes[k].nt == gnt && T->classes[k].d == (int)d) cls = k;
           if (cls < 0) {
             if (T->classes.size() >= SDIM_MAX_CLASSES) return fail(SDIM_EINVAL, "too many launch classes");
-            LaunchClass L{false, gnt, lds, 1, K};
+            LaunchClass L{false, gnt, lds, 1, K, (int)d};
             T->classes.push_back(L);
             cls = (int)T->classes.size() - 1;
           }

@@ -124,8 +124,8 @@ class Ticket:
 
     def counters(self):
         """Raw device counters: bytes (HBM-resident, LDS-resident) and phase timer ticks."""
-        buf = (ctypes.c_uint64 * 14)()
-        m = self.engine.L.sdim_ticket_counters(self.engine.ctx, self.id, buf, 14)
+        buf = (ctypes.c_uint64 * 18)()
+        m = self.engine.L.sdim_ticket_counters(self.engine.ctx, self.id, buf, 18)
         if m < 0:
             _lib.check(m)
         return [int(buf[i]) for i in range(m)]
