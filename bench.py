@@ -13,6 +13,9 @@ Besides the headline (batched persistent path) the line carries:
                  the circuits (every k-th circuit), and the batched path on the same subset
   roofline       HBM roofline of the kernel pass over HBM-resident tableaux
   roofline_lds   LDS roofline of the kernel pass over LDS-resident tableaux
+  large_tableau  the large-tableau configuration (d=5, n=8192, one circuit on a gang
+                 of workgroups) with its HBM roofline
+  serving        Poisson arrivals with HBM-budget admission: p50/p99 latency, aggregate rate
   cpu_baseline   the CPU oracle (literal restatement of the reference's
                  Program.simulate tableau path, same interpreted cost structure),
                  timed on a bounded sample on this host, 1 core
@@ -44,6 +47,7 @@ def parse():
     ap.add_argument("--shots-per-item", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="requests/s (0: 60%% of a measured burst rate)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -51,6 +55,19 @@ def parse():
                     help="run only a plain elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB) to calibrate "
                          "rocprof FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    plain copy) in profiles/<name>.md, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name + ".md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration():
@@ -135,6 +152,32 @@ def serving(eng, n_requests, rate, budget_gb, seed=5):
             "workload": "Poisson arrivals; n log-U[8,2048], d in {2,3,5,7,11}, depth 8n, shots log-U[1,4096]"}
 
 
+def large_tableau(eng, steps, warmup):
+    """d=5, n=8192: 4n {H,P,SUM} + 16 x (n/2 SUM + 64 M), 32 shots; one item on a gang of workgroups."""
+    from sdim_amd import workloads as W
+    wl = W.large_tableau()
+    tk = eng.submit(wl.jobs, "batched").wait()
+    st = tk.stats()
+    if warmup:
+        tk.replay(warmup)
+    t0 = time.perf_counter()
+    ms = tk.replay(steps) / steps
+    wall = (time.perf_counter() - t0) / steps
+    st2 = tk.stats()
+    members = tk.info("gang_members")
+    tk.release()
+    ach = st2["bytes_hbm"] / (ms * 1e-3) / 1e9
+    return {"workload": wl.name, "description": wl.description, "gate_shots": wl.gate_shots,
+            "value": wl.gate_shots / wall, "unit": "gate-shots/s", "ms_per_step": 1e3 * wall,
+            "device_ms_per_step": ms, "gang_members": members,
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": profiled_traffic("gang_kernel_large_tableau"),
+                         "traffic_source": "profiles/gang_kernel_large_tableau.md",
+                         "kernel": "sdim::gang_kernel<5, %d>" % eng.get_option("gang_threads"),
+                         "algorithmic_bytes": st2["bytes_hbm"], "kernel_ms": round(ms, 3)}}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -184,9 +227,12 @@ def main():
             return None
         c = classes[cls]
         ach = c["bytes"] / (c["ms"] * 1e-3) / 1e9
+        traffic = profiled_traffic("batch_kernel_uneven_batch") if cls == "hbm" and args.workload == "uneven" else None
         return {"bound": "hbm" if cls == "hbm" else "lds", "achieved": round(ach, 1), "peak": peak,
-                "unit": "GB/s", "frac": round(ach / peak, 4), "traffic": None,
-                "kernel": "sdim::batch_kernel<%d>" % eng.get_option("threads"),
+                "unit": "GB/s", "frac": round(ach / peak, 4), "traffic": traffic,
+                "traffic_source": "profiles/batch_kernel_uneven_batch.md" if traffic else None,
+                "kernel": ("sdim::batch_kernel<%d>" % (eng.get_option("threads") or eng.get_option("hbm_threads"))
+                           if cls == "hbm" else "sdim::batch_kernel<128..1024> (LDS-resident classes)"),
                 "items": c["items"], "algorithmic_bytes": c["bytes"], "kernel_ms": round(c["ms"], 3)}
 
     out = {
@@ -225,6 +271,9 @@ def main():
                         "batched_same_subset": gs / t_b}
         out["speedup_vs_plain"] = value / (gs / t_plain)
         out["speedup_vs_plain_same_subset"] = (gs / t_b) / (gs / t_plain)
+
+    if not args.no_large and args.workload == "uneven":
+        out["large_tableau"] = large_tableau(eng, max(1, min(args.steps, 5)), 1)
 
     if not args.no_serve:
         out["serving"] = serving(eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)

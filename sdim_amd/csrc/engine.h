@@ -8,8 +8,9 @@ namespace sdim {
 
 // number of optional phase timers after the two byte counters
 #define SDIM_NUM_TIMERS 16
-// launch classes (one item queue each)
+// launch classes, and item queues (one per launch)
 #define SDIM_MAX_CLASSES 32
+#define SDIM_MAX_QUEUES 64
 
 struct KParams {
   const ItemDesc* items;

@@ -48,6 +48,20 @@
 
 namespace sdim {
 
+// Measurement variants (compile-time switches, for measuring their effect):
+//   SDIM_EARLY_DRAWS  draw random outcomes during step A instead of step E
+//   SDIM_BALLOT_TR    compact the touched-row list with wave ballots instead of per-row LDS atomics
+//   SDIM_DET_BATCH    words per load batch in the deterministic scan
+#ifndef SDIM_EARLY_DRAWS
+#define SDIM_EARLY_DRAWS 1
+#endif
+#ifndef SDIM_BALLOT_TR
+#define SDIM_BALLOT_TR 0
+#endif
+#ifndef SDIM_DET_BATCH
+#define SDIM_DET_BATCH 1
+#endif
+
 // ------------------------------------------------------------------ scratch carve
 struct Scr {
   int* misc;         // two counter sets of 16 ints (double-buffered per measurement)
@@ -507,6 +521,233 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   return k;
 }
 
+// ------------------------------------------------------------------ rowsum steps on 16-byte column groups
+// Shared by the one-workgroup measurement and the gang measurement.  The view V
+// covers column words [w0, w0 + V.nw) of every row (V.nw a multiple of the words per
+// 16-byte group); LDS arrays (F, DOT, FL) are indexed by the view's own words.
+
+// Step A: groups of row q -> rowsum factors f = -x[q, .] in F, DOT cleared, FL = groups
+// with some f != 0 (count in M[2]), pivot candidate over the stabilizer columns
+// (atomic-min into M[0]).  rq != nullptr also keeps a copy of row q.
+template <int D, int NT>
+SDIM_DEV void rowq_groups(const Tab& V, uint32_t w0, uint32_t q, const Scr& S, int* M, uint8_t* rq,
+                          unsigned long long& bytes) {
+  constexpr bool BIN = (D == 2);
+  constexpr uint32_t CPW = BIN ? 64 : 4;
+  constexpr uint32_t WPG = BIN ? 2 : 4;              // words per 16-byte group
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n = V.n, rs = V.rs, nw = V.nw;
+  const uint32_t nsw = (n + CPW - 1) / CPW;
+  const uint32_t ngr = nw / WPG;
+  for (uint32_t g = tid; g < ngr; g += NT) {
+    const uint4 v4 = *(const uint4*)(V.X + (size_t)q * rs + 16 * g);
+    if (rq) *(uint4*)(rq + 16 * g) = v4;
+    uint4 f4;
+    bool any;
+    if constexpr (BIN) {
+      const uint64_t v0 = (uint64_t)v4.x | ((uint64_t)v4.y << 32), v1 = (uint64_t)v4.z | ((uint64_t)v4.w << 32);
+      f4 = v4;                                        // f = -x = x mod 2
+      any = (v0 | v1) != 0;
+      #pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const uint32_t gw = w0 + 2 * g + k;
+        uint64_t vs = k ? v1 : v0;
+        if (gw < nsw && vs) {
+          const uint32_t lim = n - 64 * gw;
+          if (lim < 64) vs &= (1ull << lim) - 1;
+          if (vs) atomicMin(&M[0], (int)(64 * gw + __builtin_ctzll(vs)));
+        }
+      }
+    } else {
+      const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+      f4 = make_uint4(negm<D>(v4.x), negm<D>(v4.y), negm<D>(v4.z), negm<D>(v4.w));
+      any = (f4.x | f4.y | f4.z | f4.w) != 0;
+      #pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t gw = w0 + 4 * g + k;
+        uint32_t vs = vv[k];
+        if (gw < nsw && vs) {
+          const uint32_t lim = n - 4 * gw;
+          if (lim < 4) vs &= (1u << (8 * lim)) - 1;
+          if (vs) atomicMin(&M[0], (int)(4 * gw + (__builtin_ctz(vs) >> 3)));
+        }
+      }
+    }
+    *(uint4*)(S.F + 16 * g) = f4;
+    *(uint4*)(S.DOT + 2 * WPG * g) = make_uint4(0, 0, 0, 0);                // two dot words per column word
+    if constexpr (!BIN) *(uint4*)(S.DOT + 2 * WPG * g + 4) = make_uint4(0, 0, 0, 0);
+    if (any) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)g;
+    bytes += 16;
+  }
+}
+
+// Step C: X += e x_p f, Z += e z_p f over (touched rows TR[0, nT)) x (groups FL[0, nF)),
+// column dots <z_i, x_p> of the old Z accumulated into DOT through LDS atomics.
+template <int D, int NT, int RB>
+SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF, uint32_t e,
+                           unsigned long long& bytes) {
+  constexpr bool BIN = (D == 2);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t rs = V.rs;
+    if (nT && nF) {
+      uint32_t NRG, j0, g0, jstep;
+      if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
+      else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
+      if (g0 < NRG) {
+        for (uint32_t j = j0; j < nF; j += jstep) {
+          const uint32_t gi = S.FL[j];
+          const uint4 f4 = *(const uint4*)(S.F + 16 * gi);
+          if constexpr (BIN) {
+            const uint64_t f0 = (uint64_t)f4.x | ((uint64_t)f4.y << 32), f1 = (uint64_t)f4.z | ((uint64_t)f4.w << 32);
+            uint64_t da0 = 0, da1 = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB];
+              uint4 xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
+                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
+                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
+                }
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  const uint64_t z0 = (uint64_t)zv[b].x | ((uint64_t)zv[b].y << 32), z1 = (uint64_t)zv[b].z | ((uint64_t)zv[b].w << 32);
+                  if (sxv[b]) {
+                    da0 ^= z0; da1 ^= z1;
+                    uint4 o = xv[b];
+                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
+                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = o;
+                    bytes += 32;
+                  }
+                  if (szv[b]) {
+                    uint4 o = zv[b];
+                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
+                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) = o;
+                    bytes += 16;
+                  }
+                  bytes += 16;
+                }
+              }
+            }
+            da0 &= f0; da1 &= f1;
+            if (da0) { atomicXor(&S.DOT[4 * gi], (uint32_t)da0); atomicXor(&S.DOT[4 * gi + 1], (uint32_t)(da0 >> 32)); }
+            if (da1) { atomicXor(&S.DOT[4 * gi + 2], (uint32_t)da1); atomicXor(&S.DOT[4 * gi + 3], (uint32_t)(da1 >> 32)); }
+          } else {
+            const uint32_t fw[4] = {f4.x, f4.y, f4.z, f4.w};
+            uint32_t de[4] = {0, 0, 0, 0}, dO[4] = {0, 0, 0, 0};
+            uint32_t cnt = 0;
+            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
+              uint32_t rr[RB], sxv[RB], szv[RB];
+              uint4 xv[RB], zv[RB];
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  rr[b] = S.TR[kk];
+                  sxv[b] = (e * S.xp[rr[b]]) % D;
+                  szv[b] = (e * S.zp[rr[b]]) % D;
+                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
+                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
+                }
+              }
+              #pragma unroll
+              for (int b = 0; b < RB; b++) {
+                const uint32_t kk = k0 + (uint32_t)b * NRG;
+                if (kk < nT) {
+                  const uint32_t zw[4] = {zv[b].x, zv[b].y, zv[b].z, zv[b].w};
+                  if (sxv[b]) {
+                    const MulTab& t = S.mtab[sxv[b]];
+                    const uint32_t xw[4] = {xv[b].x, xv[b].y, xv[b].z, xv[b].w};
+                    uint32_t ow[4];
+                    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                      const uint32_t pz = mult<D>(t, zw[k]);
+                      de[k] += pz & 0x00FF00FFu;
+                      dO[k] += (pz >> 8) & 0x00FF00FFu;
+                      ow[k] = addm<D>(xw[k], mult<D>(t, fw[k]));
+                    }
+                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+                    bytes += 32;
+                  }
+                  if (szv[b]) {
+                    const MulTab& t = S.mtab[szv[b]];
+                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) =
+                        make_uint4(addm<D>(zw[0], mult<D>(t, fw[0])), addm<D>(zw[1], mult<D>(t, fw[1])),
+                                   addm<D>(zw[2], mult<D>(t, fw[2])), addm<D>(zw[3], mult<D>(t, fw[3])));
+                    bytes += 16;
+                  }
+                  bytes += 16;
+                }
+              }
+              if (++cnt == 512) {
+                #pragma unroll
+                for (int k = 0; k < 4; k++) { de[k] = fold16<D>(de[k]); dO[k] = fold16<D>(dO[k]); }
+                cnt = 0;
+              }
+            }
+            #pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const uint32_t a = fold16<D>(de[k]), b = fold16<D>(dO[k]);
+              if (a) atomicAdd(&S.DOT[2 * (4 * gi + k)], a);
+              if (b) atomicAdd(&S.DOT[2 * (4 * gi + k) + 1], b);
+            }
+          }
+        }
+      }
+    }
+}
+
+// Step D (phases): ph_i += f_i ph_p' + f_i <z_i, x_p> + xz f_i (f_i - 1) / 2 for every shot and
+// every word of the groups in FL (ph_p' = exponentiated pivot phase of the shot).
+template <int D, int NT>
+SDIM_DEV void rowsum_phases(const Tab& V, const Scr& S, uint32_t nF, uint32_t e, uint32_t xz, uint32_t pinc,
+                            unsigned long long& bytes) {
+  constexpr bool BIN = (D == 2);
+  constexpr uint32_t WPG = BIN ? 2 : 4;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t Sh = V.S;
+    for (uint32_t i = tid; i < Sh * nF * WPG; i += NT) {
+      const uint32_t s = i / (nF * WPG), jk = i - s * nF * WPG;
+      const uint32_t w = S.FL[jk / WPG] * WPG + jk % WPG;
+      if constexpr (BIN) {
+        const uint32_t k2 = S.shot[s];
+        const uint64_t f = ((const uint64_t*)S.F)[w];
+        if (!f) continue;
+        const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
+        uint64_t* p0 = plane(V, s, 0) + w;
+        uint64_t* p1 = plane(V, s, 1) + w;
+        uint64_t a0 = *p0, a1 = *p1;
+        const uint64_t k0m = (k2 & 1) ? f : 0ull, k1m = (k2 & 2) ? f : 0ull;
+        a1 ^= k1m ^ (a0 & k0m);
+        a0 ^= k0m;
+        a1 ^= cdot;
+        *p0 = a0; *p1 = a1;
+        bytes += 32;
+      } else {
+        const uint32_t f = ((const uint32_t*)S.F)[w];
+        if (!f) continue;
+        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
+        uint32_t c = 0;
+        #pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+          const uint32_t fb = byte_of(f, kb);
+          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
+          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
+        }
+        const uint32_t php = (e * S.shot[s] + pinc) % D;
+        uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
+        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
+        bytes += 8;
+      }
+    }
+}
+
 // ------------------------------------------------------------------ measurement
 // Z-basis measurement of qudit q (tableau_prime.py:262-363) followed, for
 // RESET, by X^((-m) mod d) (program.py:334-339).  Writes one record per shot.
@@ -516,7 +757,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
                       const Scr& S, const ItemDesc& it, const KParams& P, unsigned long long& bytes) {
   constexpr bool BIN = (D == 2);
   constexpr uint32_t CPW = BIN ? 64 : 4;
-  constexpr int RB = 4;     // rows per load batch in the sweep
+  constexpr int RB = 2;     // rows per load batch in the sweep (16 bytes of X and Z each)
   const uint32_t tid = threadIdx.x;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t G = 2 * n;
@@ -527,37 +768,12 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
   uint8_t* rec = P.arena + it.rec_off + (size_t)slot * Sh;
   SDIM_TMARK(tA);
 
-  // ---- A: row q -> pivot candidate, f words, list of words with f != 0
-  for (uint32_t w = tid; w < NWd; w += NT) {
-    S.DOT[2 * w] = 0;
-    S.DOT[2 * w + 1] = 0;
-    if constexpr (BIN) {
-      const uint64_t v = row64(T.X, q, rs)[w];
-      ((uint64_t*)S.RQ)[w] = v;
-      ((uint64_t*)S.F)[w] = v;                      // f = -x = x mod 2
-      if (v) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
-      if (w < nsw) {
-        uint64_t vs = v;
-        const uint32_t lim = n - 64 * w;
-        if (lim < 64) vs &= (1ull << lim) - 1;
-        if (vs) atomicMin(&M[0], (int)(64 * w + __builtin_ctzll(vs)));
-      }
-      bytes += 8;
-    } else {
-      const uint32_t v = row32(T.X, q, rs)[w];
-      ((uint32_t*)S.RQ)[w] = v;
-      const uint32_t f = negm<D>(v);
-      ((uint32_t*)S.F)[w] = f;
-      if (f) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)w;
-      if (w < nsw) {
-        uint32_t vs = v;
-        const uint32_t lim = n - 4 * w;
-        if (lim < 4) vs &= (1u << (8 * lim)) - 1;
-        if (vs) atomicMin(&M[0], (int)(4 * w + (__builtin_ctz(vs) >> 3)));
-      }
-      bytes += 4;
-    }
-  }
+  // random outcomes do not depend on the tableau: draw them first (step E uses them when
+  // the measurement turns out to be random)
+  if constexpr (SDIM_EARLY_DRAWS)
+    for (uint32_t s = tid; s < Sh; s += NT) S.shot[Sh + s] = outcome_of(draw(it.seed, it.stream, src, it.shot0 + s), D);
+  // ---- A: row q -> pivot candidate, f, groups with f != 0, copy of row q
+  rowq_groups<D, NT>(T, 0, q, S, M, S.RQ, bytes);
   __syncthreads();                                                        // B1
   SDIM_TADD(P, 0, tA);
   SDIM_TMARK(tB);
@@ -566,24 +782,65 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 
   if (piv != INT_MAX) {
     const uint32_t p = (uint32_t)piv;
-    // ---- B: gather the pivot column, touched rows, <x_p, z_p>, pivot phases
-    for (uint32_t r = tid; r < n; r += NT) {
-      uint32_t xr, zr;
-      if constexpr (BIN) {
-        xr = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-        zr = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
-        bytes += 16;
-      } else {
-        xr = T.X[(size_t)r * rs + p];
-        zr = T.Z[(size_t)r * rs + p];
-        bytes += 2;
+    // ---- B: gather the pivot column, touched rows (compacted per wave), <x_p, z_p>, pivot phases
+    if constexpr (!SDIM_BALLOT_TR) {
+      for (uint32_t r = tid; r < n; r += NT) {
+        uint32_t xr, zr;
+        if constexpr (BIN) {
+          xr = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
+          zr = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+          bytes += 16;
+        } else {
+          xr = T.X[(size_t)r * rs + p];
+          zr = T.Z[(size_t)r * rs + p];
+          bytes += 2;
+        }
+        S.xp[r] = (uint8_t)xr;
+        S.zp[r] = (uint8_t)zr;
+        if (xr | zr) {
+          S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
+          if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
+        }
       }
-      S.xp[r] = (uint8_t)xr;
-      S.zp[r] = (uint8_t)zr;
-      if (xr | zr) {
-        S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
-        if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
+    } else {
+      constexpr int GB = 4;
+      const uint32_t lane = tid & 63;
+      uint32_t xzs = 0;
+      for (uint32_t r0 = 0; r0 < n; r0 += NT * GB) {
+        uint32_t xs[GB], zs[GB];
+        #pragma unroll
+        for (int b = 0; b < GB; b++) {
+          const uint32_t r = r0 + (uint32_t)b * NT + tid;
+          xs[b] = 0; zs[b] = 0;
+          if (r < n) {
+            if constexpr (BIN) {
+              xs[b] = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
+              zs[b] = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+              bytes += 16;
+            } else {
+              xs[b] = T.X[(size_t)r * rs + p];
+              zs[b] = T.Z[(size_t)r * rs + p];
+              bytes += 2;
+            }
+          }
+        }
+        #pragma unroll
+        for (int b = 0; b < GB; b++) {
+          const uint32_t r = r0 + (uint32_t)b * NT + tid;
+          const bool hit = (xs[b] | zs[b]) != 0;
+          const uint64_t bal = __ballot(hit);
+          uint32_t base = 0;
+          if (lane == 0 && bal) base = (uint32_t)atomicAdd(&M[1], (int)__popcll(bal));
+          base = __shfl(base, 0, 64) + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+          if (r < n) {
+            S.xp[r] = (uint8_t)xs[b];
+            S.zp[r] = (uint8_t)zs[b];
+            if (hit) S.TR[base] = (uint16_t)r;
+            xzs += xs[b] * zs[b];
+          }
+        }
       }
+      if constexpr (!BIN) if (xzs) atomicAdd(&M[3], (int)(xzs % D));
     }
     if (tid == 0) {
       if constexpr (BIN) {
@@ -619,86 +876,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
     }
 
-    // ---- C: sweep (touched rows) x (words with f != 0)
-    if (nT && nF) {
-      uint32_t NRG, j0, g0, jstep;
-      if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
-      else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
-      if (g0 < NRG) {
-        for (uint32_t j = j0; j < nF; j += jstep) {
-          const uint32_t w = S.FL[j];
-          if constexpr (BIN) {
-            const uint64_t f = ((const uint64_t*)S.F)[w];
-            uint64_t dacc = 0;
-            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
-              uint32_t rr[RB], sxv[RB], szv[RB];
-              uint64_t xv[RB], zv[RB];
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  rr[b] = S.TR[kk];
-                  sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
-                  zv[b] = row64(T.Z, rr[b], rs)[w];
-                  xv[b] = sxv[b] ? row64(T.X, rr[b], rs)[w] : 0ull;
-                }
-              }
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  if (sxv[b]) { dacc ^= zv[b]; row64(T.X, rr[b], rs)[w] = xv[b] ^ f; bytes += 16; }
-                  if (szv[b]) { row64(T.Z, rr[b], rs)[w] = zv[b] ^ f; bytes += 8; }
-                  bytes += 8;
-                }
-              }
-            }
-            if (dacc) {
-              atomicXor(&S.DOT[2 * w], (uint32_t)dacc);
-              atomicXor(&S.DOT[2 * w + 1], (uint32_t)(dacc >> 32));
-            }
-          } else {
-            const uint32_t f = ((const uint32_t*)S.F)[w];
-            uint32_t de = 0, dO = 0, cnt = 0;
-            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
-              uint32_t rr[RB], sxv[RB], szv[RB], xv[RB], zv[RB];
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  rr[b] = S.TR[kk];
-                  sxv[b] = (e * S.xp[rr[b]]) % D;
-                  szv[b] = (e * S.zp[rr[b]]) % D;
-                  zv[b] = row32(T.Z, rr[b], rs)[w];
-                  xv[b] = sxv[b] ? row32(T.X, rr[b], rs)[w] : 0u;
-                }
-              }
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  if (sxv[b]) {
-                    const MulTab& t = S.mtab[sxv[b]];
-                    const uint32_t pz = mult<D>(t, zv[b]);
-                    de += pz & 0x00FF00FFu;
-                    dO += (pz >> 8) & 0x00FF00FFu;
-                    row32(T.X, rr[b], rs)[w] = addm<D>(xv[b], mult<D>(t, f));
-                    bytes += 8;
-                  }
-                  if (szv[b]) { row32(T.Z, rr[b], rs)[w] = addm<D>(zv[b], mult<D>(S.mtab[szv[b]], f)); bytes += 4; }
-                  bytes += 4;
-                }
-              }
-              if (++cnt == 1024) { de = fold16<D>(de); dO = fold16<D>(dO); cnt = 0; }
-            }
-            de = fold16<D>(de);
-            dO = fold16<D>(dO);
-            if (de) atomicAdd(&S.DOT[2 * w], de);
-            if (dO) atomicAdd(&S.DOT[2 * w + 1], dO);
-          }
-        }
-      }
-    }
+    // ---- C: sweep (touched rows) x (groups with f != 0)
+    sweep_groups<D, NT, RB>(T, S, nT, nF, e, bytes);
     __syncthreads();                                                      // B3
     SDIM_TADD(P, 2, tC);
     SDIM_TMARK(tD);
@@ -732,45 +911,15 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         bytes += 4;
       }
     }
-    for (uint32_t i = tid; i < Sh * nF; i += NT) {
-      const uint32_t s = i / nF, j = i - s * nF;
-      const uint32_t w = S.FL[j];
-      if constexpr (BIN) {
-        const uint32_t k2 = S.shot[s];
-        const uint64_t f = ((const uint64_t*)S.F)[w];
-        const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
-        uint64_t* p0 = plane(T, s, 0) + w;
-        uint64_t* p1 = plane(T, s, 1) + w;
-        uint64_t a0 = *p0, a1 = *p1;
-        const uint64_t k0m = (k2 & 1) ? f : 0ull, k1m = (k2 & 2) ? f : 0ull;
-        a1 ^= k1m ^ (a0 & k0m);
-        a0 ^= k0m;
-        a1 ^= cdot;                                   // po * f * <z_i, x_p>
-        *p0 = a0; *p1 = a1;
-        bytes += 32;
-      } else {
-        const uint32_t f = ((const uint32_t*)S.F)[w];
-        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
-        uint32_t c = 0;
-        #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-          const uint32_t fb = byte_of(f, kb);
-          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
-          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
-        }
-        const uint32_t php = (e * S.shot[s] + pinc) % D;   // exponentiated pivot phase
-        uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
-        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
-        bytes += 8;
-      }
-    }
+    rowsum_phases<D, NT>(T, S, nF, e, xz, pinc, bytes);
     __syncthreads();                                                      // B4
     SDIM_TADD(P, 3, tD);
     SDIM_TMARK(tE);
-    // ---- E: outcome per shot; ph_{n+p} <- ph_p, ph_p <- -m po
+    // ---- E: outcome per shot (drawn in step A); ph_{n+p} <- ph_p, ph_p <- -m po
     for (uint32_t s = tid; s < Sh; s += NT) {
-      const U64x4 o = draw(it.seed, it.stream, src, it.shot0 + s);
-      const uint32_t m = outcome_of(o, D);
+      uint32_t m;
+      if constexpr (SDIM_EARLY_DRAWS) m = S.shot[Sh + s];
+      else { m = outcome_of(draw(it.seed, it.stream, src, it.shot0 + s), D); S.shot[Sh + s] = m; }
       if constexpr (BIN) {
         const uint32_t k2 = S.shot[s];
         const uint32_t g = n + p;
@@ -785,7 +934,6 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         ph[n + p] = (uint8_t)((e * S.shot[s] + pinc) % D);
         ph[p] = (uint8_t)((D - m) % D);
       }
-      S.shot[Sh + s] = m;
       rec[s] = (uint8_t)m;
     }
     SDIM_TADD(P, 4, tE);
@@ -816,40 +964,65 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     __syncthreads();
     // cross term sum_r sum_i (f x)_ri * prefix_{j<i} (f z)_rj and the exponent term
     // sum_i f_i (f_i - 1) / 2 <x_i, z_i>, one thread per row
+    // (loads are issued DB words ahead of their use; words with f = 0 are skipped)
+    constexpr uint32_t DB = SDIM_DET_BATCH;
     uint32_t part = 0;
     for (uint32_t r = tid; r < n; r += NT) {
       if constexpr (BIN) {
         const uint64_t* xr = row64(T.X, r, rs);
         const uint64_t* zr = row64(T.Z, r, rs);
         uint64_t carry = 0, par = 0;
-        for (uint32_t w = 0; w < nsw; w++) {
-          const uint64_t f = ((const uint64_t*)S.F)[w];
-          if (!f) continue;
-          const uint64_t a = xr[w] & f, b = zr[w] & f;
-          uint64_t y = b;
-          y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
-          const uint64_t pre = (y << 1) ^ (carry ? ~0ull : 0ull);
-          par ^= (uint64_t)(__builtin_popcountll(a & pre) & 1);
-          carry ^= (uint64_t)(__builtin_popcountll(b) & 1);
-          bytes += 16;
+        for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
+          uint64_t xa[DB], za[DB];
+          #pragma unroll
+          for (uint32_t u = 0; u < DB; u++) {
+            const uint32_t w = w0 + u;
+            const uint64_t f = w < nsw ? ((const uint64_t*)S.F)[w] : 0ull;
+            xa[u] = f ? xr[w] : 0ull;
+            za[u] = f ? zr[w] : 0ull;
+          }
+          #pragma unroll
+          for (uint32_t u = 0; u < DB; u++) {
+            const uint32_t w = w0 + u;
+            const uint64_t f = w < nsw ? ((const uint64_t*)S.F)[w] : 0ull;
+            if (!f) continue;
+            const uint64_t a = xa[u] & f, b = za[u] & f;
+            uint64_t y = b;
+            y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
+            const uint64_t pre = (y << 1) ^ (carry ? ~0ull : 0ull);
+            par ^= (uint64_t)(__builtin_popcountll(a & pre) & 1);
+            carry ^= (uint64_t)(__builtin_popcountll(b) & 1);
+            bytes += 16;
+          }
         }
         part += (uint32_t)par;
       } else {
         const uint32_t* xr = row32(T.X, r, rs);
         const uint32_t* zr = row32(T.Z, r, rs);
         uint32_t U = 0, acc = 0;
-        for (uint32_t w = 0; w < nsw; w++) {
-          const uint32_t f = ((const uint32_t*)S.F)[w];
-          if (!f) continue;
-          const uint32_t x = xr[w], z = zr[w];
+        for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
+          uint32_t xa[DB], za[DB];
           #pragma unroll
-          for (int b = 0; b < 4; b++) {
-            const uint32_t fb = byte_of(f, b);
-            const uint32_t xb = byte_of(x, b), zb = byte_of(z, b);
-            acc += fb * xb * U + ((fb * (fb - 1) / 2) % D) * xb * zb;
-            U = (U + fb * zb) % D;
+          for (uint32_t u = 0; u < DB; u++) {
+            const uint32_t w = w0 + u;
+            const uint32_t f = w < nsw ? ((const uint32_t*)S.F)[w] : 0u;
+            xa[u] = f ? xr[w] : 0u;
+            za[u] = f ? zr[w] : 0u;
           }
-          bytes += 8;
+          #pragma unroll
+          for (uint32_t u = 0; u < DB; u++) {
+            const uint32_t w = w0 + u;
+            const uint32_t f = w < nsw ? ((const uint32_t*)S.F)[w] : 0u;
+            if (!f) continue;
+            #pragma unroll
+            for (int b = 0; b < 4; b++) {
+              const uint32_t fb = byte_of(f, b);
+              const uint32_t xb = byte_of(xa[u], b), zb = byte_of(za[u], b);
+              acc += fb * xb * U + ((fb * (fb - 1) / 2) % D) * xb * zb;
+              U = (U + fb * zb) % D;
+            }
+            bytes += 8;
+          }
         }
         part += acc % D;
       }
@@ -977,7 +1150,7 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
 }
 
 // ------------------------------------------------------------------ persistent batch kernel
-// Each workgroup pulls items from P.queue until the queue is drained.
+// Each workgroup pulls items (of any dimension) from P.queue until the queue is drained.
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(KParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -1152,47 +1325,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
   // phase timers (gang): 0 A, 1 barrier 1, 2 column pack, 3 barrier 2, 4 unpack, 5 sweep, 6 D, 10 E, 12 det
   SDIM_TMARK(tA);
   // ---- A: own 16-byte groups of row q -> pivot candidate, f, groups with f != 0
-  constexpr uint32_t WPG = BIN ? 2 : 4;              // words per 16-byte group
-  const uint32_t ngr = nw / WPG;
-  for (uint32_t g = tid; g < ngr; g += NT) {
-    const uint4 v4 = *(const uint4*)(V.X + (size_t)q * rs + 16 * g);
-    uint4 f4;
-    bool any;
-    if constexpr (BIN) {
-      const uint64_t v0 = (uint64_t)v4.x | ((uint64_t)v4.y << 32), v1 = (uint64_t)v4.z | ((uint64_t)v4.w << 32);
-      f4 = v4;                                        // f = -x = x mod 2
-      any = (v0 | v1) != 0;
-      #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const uint32_t gw = w0 + 2 * g + k;
-        uint64_t vs = k ? v1 : v0;
-        if (gw < nsw && vs) {
-          const uint32_t lim = n - 64 * gw;
-          if (lim < 64) vs &= (1ull << lim) - 1;
-          if (vs) atomicMin(&M[0], (int)(64 * gw + __builtin_ctzll(vs)));
-        }
-      }
-    } else {
-      const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
-      f4 = make_uint4(negm<D>(v4.x), negm<D>(v4.y), negm<D>(v4.z), negm<D>(v4.w));
-      any = (f4.x | f4.y | f4.z | f4.w) != 0;
-      #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const uint32_t gw = w0 + 4 * g + k;
-        uint32_t vs = vv[k];
-        if (gw < nsw && vs) {
-          const uint32_t lim = n - 4 * gw;
-          if (lim < 4) vs &= (1u << (8 * lim)) - 1;
-          if (vs) atomicMin(&M[0], (int)(4 * gw + (__builtin_ctz(vs) >> 3)));
-        }
-      }
-    }
-    *(uint4*)(S.F + 16 * g) = f4;
-    *(uint4*)(S.DOT + 2 * WPG * g) = make_uint4(0, 0, 0, 0);                // two dot words per column word
-    if constexpr (!BIN) *(uint4*)(S.DOT + 2 * WPG * g + 4) = make_uint4(0, 0, 0, 0);
-    if (any) S.FL[atomicAdd(&M[2], 1)] = (uint16_t)g;
-    bytes += 16;
-  }
+  rowq_groups<D, NT>(V, w0, q, S, M, nullptr, bytes);
   __syncthreads();
   if (tid == 0 && M[0] != INT_MAX)
     __hip_atomic_fetch_min(&Gc.sy->piv[mk & 1], M[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1318,118 +1451,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     }
 
     // ---- C: sweep (touched rows) x (own 16-byte groups with f != 0)
-    if (nT && nF) {
-      uint32_t NRG, j0, g0, jstep;
-      if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
-      else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
-      if (g0 < NRG) {
-        for (uint32_t j = j0; j < nF; j += jstep) {
-          const uint32_t gi = S.FL[j];
-          const uint4 f4 = *(const uint4*)(S.F + 16 * gi);
-          if constexpr (BIN) {
-            const uint64_t f0 = (uint64_t)f4.x | ((uint64_t)f4.y << 32), f1 = (uint64_t)f4.z | ((uint64_t)f4.w << 32);
-            uint64_t da0 = 0, da1 = 0;
-            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
-              uint32_t rr[RB], sxv[RB], szv[RB];
-              uint4 xv[RB], zv[RB];
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  rr[b] = S.TR[kk];
-                  sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
-                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
-                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
-                }
-              }
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  const uint64_t z0 = (uint64_t)zv[b].x | ((uint64_t)zv[b].y << 32), z1 = (uint64_t)zv[b].z | ((uint64_t)zv[b].w << 32);
-                  if (sxv[b]) {
-                    da0 ^= z0; da1 ^= z1;
-                    uint4 o = xv[b];
-                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
-                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = o;
-                    bytes += 32;
-                  }
-                  if (szv[b]) {
-                    uint4 o = zv[b];
-                    o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
-                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) = o;
-                    bytes += 16;
-                  }
-                  bytes += 16;
-                }
-              }
-            }
-            da0 &= f0; da1 &= f1;
-            if (da0) { atomicXor(&S.DOT[4 * gi], (uint32_t)da0); atomicXor(&S.DOT[4 * gi + 1], (uint32_t)(da0 >> 32)); }
-            if (da1) { atomicXor(&S.DOT[4 * gi + 2], (uint32_t)da1); atomicXor(&S.DOT[4 * gi + 3], (uint32_t)(da1 >> 32)); }
-          } else {
-            const uint32_t fw[4] = {f4.x, f4.y, f4.z, f4.w};
-            uint32_t de[4] = {0, 0, 0, 0}, dO[4] = {0, 0, 0, 0};
-            uint32_t cnt = 0;
-            for (uint32_t k0 = g0; k0 < nT; k0 += NRG * RB) {
-              uint32_t rr[RB], sxv[RB], szv[RB];
-              uint4 xv[RB], zv[RB];
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  rr[b] = S.TR[kk];
-                  sxv[b] = (e * S.xp[rr[b]]) % D;
-                  szv[b] = (e * S.zp[rr[b]]) % D;
-                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
-                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
-                }
-              }
-              #pragma unroll
-              for (int b = 0; b < RB; b++) {
-                const uint32_t kk = k0 + (uint32_t)b * NRG;
-                if (kk < nT) {
-                  const uint32_t zw[4] = {zv[b].x, zv[b].y, zv[b].z, zv[b].w};
-                  if (sxv[b]) {
-                    const MulTab& t = S.mtab[sxv[b]];
-                    const uint32_t xw[4] = {xv[b].x, xv[b].y, xv[b].z, xv[b].w};
-                    uint32_t ow[4];
-                    #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                      const uint32_t pz = mult<D>(t, zw[k]);
-                      de[k] += pz & 0x00FF00FFu;
-                      dO[k] += (pz >> 8) & 0x00FF00FFu;
-                      ow[k] = addm<D>(xw[k], mult<D>(t, fw[k]));
-                    }
-                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
-                    bytes += 32;
-                  }
-                  if (szv[b]) {
-                    const MulTab& t = S.mtab[szv[b]];
-                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) =
-                        make_uint4(addm<D>(zw[0], mult<D>(t, fw[0])), addm<D>(zw[1], mult<D>(t, fw[1])),
-                                   addm<D>(zw[2], mult<D>(t, fw[2])), addm<D>(zw[3], mult<D>(t, fw[3])));
-                    bytes += 16;
-                  }
-                  bytes += 16;
-                }
-              }
-              if (++cnt == 512) {
-                #pragma unroll
-                for (int k = 0; k < 4; k++) { de[k] = fold16<D>(de[k]); dO[k] = fold16<D>(dO[k]); }
-                cnt = 0;
-              }
-            }
-            #pragma unroll
-            for (int k = 0; k < 4; k++) {
-              const uint32_t a = fold16<D>(de[k]), b = fold16<D>(dO[k]);
-              if (a) atomicAdd(&S.DOT[2 * (4 * gi + k)], a);
-              if (b) atomicAdd(&S.DOT[2 * (4 * gi + k) + 1], b);
-            }
-          }
-        }
-      }
-    }
+    sweep_groups<D, NT, RB>(V, S, nT, nF, e, bytes);
     __syncthreads();
     SDIM_TADD(P, 5, tC);
     SDIM_TMARK(tD);
@@ -1473,40 +1495,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         }
       }
     }
-    for (uint32_t i = tid; i < Sh * nF * WPG; i += NT) {
-      const uint32_t s = i / (nF * WPG), jk = i - s * nF * WPG;
-      const uint32_t w = S.FL[jk / WPG] * WPG + jk % WPG;
-      if constexpr (BIN) {
-        const uint32_t k2 = S.shot[s];
-        const uint64_t f = ((const uint64_t*)S.F)[w];
-        if (!f) continue;
-        const uint64_t cdot = ((uint64_t)S.DOT[2 * w] | ((uint64_t)S.DOT[2 * w + 1] << 32)) & f;
-        uint64_t* p0 = plane(V, s, 0) + w;
-        uint64_t* p1 = plane(V, s, 1) + w;
-        uint64_t a0 = *p0, a1 = *p1;
-        const uint64_t k0m = (k2 & 1) ? f : 0ull, k1m = (k2 & 2) ? f : 0ull;
-        a1 ^= k1m ^ (a0 & k0m);
-        a0 ^= k0m;
-        a1 ^= cdot;
-        *p0 = a0; *p1 = a1;
-        bytes += 32;
-      } else {
-        const uint32_t f = ((const uint32_t*)S.F)[w];
-        if (!f) continue;
-        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
-        uint32_t c = 0;
-        #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-          const uint32_t fb = byte_of(f, kb);
-          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
-          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
-        }
-        const uint32_t php = (e * S.shot[s] + pinc) % D;
-        uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
-        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
-        bytes += 8;
-      }
-    }
+    rowsum_phases<D, NT>(V, S, nF, e, xz, pinc, bytes);
     __syncthreads();
     SDIM_TADD(P, 6, tD);
     SDIM_TMARK(tE);
@@ -1900,7 +1889,7 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
 namespace sdim {
 
 __global__ void reset_kernel(uint32_t* queue, unsigned long long* counters) {
-  if (threadIdx.x < SDIM_MAX_CLASSES) queue[threadIdx.x] = 0;
+  if (threadIdx.x < SDIM_MAX_QUEUES) queue[threadIdx.x] = 0;
   if (threadIdx.x < 2 + SDIM_NUM_TIMERS) counters[threadIdx.x] = 0;
 }
 

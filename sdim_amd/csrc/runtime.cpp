@@ -104,7 +104,10 @@ std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
 }
 
 struct Group {
+  int branch = 0;      // which parallel branch launches it (batch groups)
+  uint64_t cost = 0;   // estimated work (LPT cost sum)
   int cls = 0;
+  int d = 0;           // gang groups: the dimension of their items (0: mixed)
   uint32_t first = 0, count = 0;
   GangParams gp{};     // gang classes: barrier records, scratch, error word
   int gang_grid = 0;   // gang classes: workgroups launched (gangs x members)
@@ -115,6 +118,8 @@ struct Ticket {
   int mode = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipStream_t> branches;                // parallel branches of the batch launches
+  std::vector<hipEvent_t> fork_join;                // [0] fork, [1..] joins
   Alloc arena, meta;
   uint64_t rec_bytes = 0;
   std::vector<JobPlan> jobs;
@@ -161,6 +166,7 @@ struct sdim_ctx {
   int64_t gang_threads = 512;
   int64_t gang_max = 0;       // most members per gang (0: every CU)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
+  std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
 };
 
 namespace {
@@ -439,6 +445,7 @@ void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::itera
   for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
+  for (auto e : t->fork_join) (void)hipEventDestroy(e);
   c->tickets.erase(it);
 }
 
@@ -453,14 +460,30 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
   KParams& P = T->P;
   if (T->mode == SDIM_MODE_BATCHED) {
     if (launch_reset(T->d_queue, P.counters, T->stream)) return fail(SDIM_EHIP, "reset launch failed");
+    // batch launches run as parallel branches (fork/join through events: parallel nodes
+    // when captured into a graph) so that the tail of one launch overlaps the others;
+    // gang launches follow the join, one at a time
+    const int nb = (int)T->branches.size();
+    std::vector<bool> used(nb + 1, false);
+    if (nb && hipEventRecord(T->fork_join[0], T->stream) != hipSuccess) return fail(SDIM_EHIP, "hipEventRecord failed");
+    for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      for (int b = 1; b <= nb; b++) {
+        if (!used[b]) continue;
+        if (hipEventRecord(T->fork_join[b], T->branches[b - 1]) != hipSuccess ||
+            hipStreamWaitEvent(T->stream, T->fork_join[b], 0) != hipSuccess)
+          return fail(SDIM_EHIP, "branch join failed");
+      }
+    }
     for (const Group& g : T->groups) {
       const LaunchClass& L = T->classes[g.cls];
       if (which == 1 && L.resident) continue;
       if (which == 2 && !L.resident) continue;
+      if ((pass == 0) == (L.gang != 0)) continue;
       KParams Q = P;
       Q.order = P.order + g.first;
       Q.n_items = g.count;
-      Q.queue = T->d_queue + g.cls;
+      Q.queue = T->d_queue + (&g - T->groups.data());
       Q.lds_bytes = std::min<uint32_t>(L.lds, P.lds_bytes);
       if (L.gang) {
         if (gang_gate && hipStreamWaitEvent(T->stream, gang_gate, 0) != hipSuccess)
@@ -473,7 +496,14 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
       const int per_cu = std::max(1, std::min(L.per_cu, 1024 / nt));
       const int grid = T->grid_override ? T->grid_override
                                         : (int)std::min<uint64_t>((uint64_t)T->n_cu * per_cu, g.count);
-      if (launch_batch(Q, nt, grid, T->stream)) return fail(SDIM_EHIP, "batch kernel launch failed");
+      const int b = std::min(g.branch, nb);
+      hipStream_t st = b ? T->branches[b - 1] : T->stream;
+      if (b && !used[b]) {
+        if (hipStreamWaitEvent(st, T->fork_join[0], 0) != hipSuccess) return fail(SDIM_EHIP, "branch fork failed");
+        used[b] = true;
+      }
+      if (launch_batch(Q, nt, grid, st)) return fail(SDIM_EHIP, "batch kernel launch failed");
+    }
     }
     return 0;
   }
@@ -525,6 +555,14 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
     c->streams.push_back(st);
   }
   c->stream = c->streams[0];
+  c->branch_streams.resize(c->streams.size());
+  for (auto& v : c->branch_streams) {
+    for (int i = 0; i < 3; i++) {
+      hipStream_t st = nullptr;
+      HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      v.push_back(st);
+    }
+  }
   HIP_OK(hipEventCreateWithFlags(&c->gang_done, hipEventDisableTiming));
   if (prepare_kernels()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
   *out = c.release();
@@ -544,9 +582,11 @@ int sdim_ctx_destroy(sdim_ctx* c) {
     for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
+    for (auto e : t->fork_join) (void)hipEventDestroy(e);
   }
   for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
   for (auto st : c->streams) (void)hipStreamDestroy(st);
+  for (auto& v : c->branch_streams) for (auto st : v) (void)hipStreamDestroy(st);
   if (c->gang_done) (void)hipEventDestroy(c->gang_done);
   delete c;
   return 0;
@@ -793,10 +833,37 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       while (e < order.size() && cls_of[order[e]] == k) e++;
       Group g;
       g.cls = k;
+      g.d = T->classes[k].d;
       g.first = i;
       g.count = e - i;
       T->groups.push_back(g);
       i = e;
+    }
+  }
+  // batch groups over parallel branches: longest first onto the least loaded branch
+  {
+    const int n_br = 1 + (int)c->branch_streams[(size_t)stream_idx % c->branch_streams.size()].size();
+    std::vector<size_t> gi;
+    for (size_t i = 0; i < T->groups.size(); i++) {
+      Group& g = T->groups[i];
+      for (uint32_t k = g.first; k < g.first + g.count; k++) g.cost += cost[order[k]];
+      if (!T->classes[g.cls].gang) gi.push_back(i);
+    }
+    std::stable_sort(gi.begin(), gi.end(), [&](size_t a, size_t b) { return T->groups[a].cost > T->groups[b].cost; });
+    std::vector<uint64_t> load((size_t)n_br, 0);
+    int used_br = 0;
+    for (size_t i : gi) {
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[(size_t)b] += T->groups[i].cost;
+      T->groups[i].branch = b;
+      used_br = std::max(used_br, b);
+    }
+    const auto& bs = c->branch_streams[(size_t)stream_idx % c->branch_streams.size()];
+    T->branches.assign(bs.begin(), bs.begin() + used_br);
+    for (int k = 0; k <= used_br && used_br > 0; k++) {
+      hipEvent_t e = nullptr;
+      HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      T->fork_join.push_back(e);
     }
   }
   // gang groups: as many gangs as fit on the device at once (and no more than items)
@@ -828,7 +895,8 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   const uint64_t o_ops = align256(o_order + order.size() * 4);
   const uint64_t o_keep = align256(o_ops + ops.size() * sizeof(DevOp));
   const uint64_t o_queue = align256(o_keep + keep.size() * 8);
-  const uint64_t o_cnt = align256(o_queue + 4 * SDIM_MAX_CLASSES);
+  if (T->groups.size() > SDIM_MAX_QUEUES) return fail(SDIM_EINVAL, "too many launch groups");
+  const uint64_t o_cnt = align256(o_queue + 4 * SDIM_MAX_QUEUES);
   const uint64_t meta_bytes = align256(o_cnt + 8 * (2 + SDIM_NUM_TIMERS));
   int rc = ctx_alloc(c, arena_bytes, T->arena);
   if (rc) return rc;
