@@ -113,10 +113,13 @@ struct Tab {
   uint32_t wofs;      // first column word of this view (column tiles of an HBM-resident tableau)
   uint32_t rs_full;   // width in bytes of the per-column LDS arrays (segment deltas, noise snapshots)
   uint32_t nw;        // column words covered by the view (gate sweeps and phase passes)
+  uint32_t il;        // X/Z row interleave (see sdim_types.h)
 };
 
-SDIM_DEV uint32_t* row32(uint8_t* base, uint32_t r, uint32_t rs) { return (uint32_t*)(base + (size_t)r * rs); }
-SDIM_DEV uint64_t* row64(uint8_t* base, uint32_t r, uint32_t rs) { return (uint64_t*)(base + (size_t)r * rs); }
+// word w (4 bytes, odd d) / word w (8 bytes, d = 2) / byte b of row r of an X/Z block
+SDIM_DEV uint32_t* w32(uint8_t* base, uint32_t r, uint32_t w, uint32_t rs, uint32_t il) { return (uint32_t*)(base + cell(r, 4 * w, rs, il)); }
+SDIM_DEV uint64_t* w64(uint8_t* base, uint32_t r, uint32_t w, uint32_t rs, uint32_t il) { return (uint64_t*)(base + cell(r, 8 * w, rs, il)); }
+SDIM_DEV uint8_t* b8(uint8_t* base, uint32_t r, uint32_t b, uint32_t rs, uint32_t il) { return base + cell(r, b, rs, il); }
 SDIM_DEV uint8_t* phase_vec(const Tab& T, uint32_t s) { return T.Ph + (size_t)s * T.rs; }
 SDIM_DEV uint64_t* plane(const Tab& T, uint32_t s, int k) { return (uint64_t*)(T.Ph + (size_t)(2 * s + k) * T.rs); }
 
@@ -177,11 +180,11 @@ SDIM_DEV void init_identity(const Tab& T, uint64_t total) {
   for (uint32_t r = threadIdx.x; r < n; r += NT) {
     if constexpr (D == 2) {
       const uint32_t g = n + r;
-      row64(T.X, r, T.rs)[g >> 6] |= 1ull << (g & 63);
-      row64(T.Z, r, T.rs)[r >> 6] |= 1ull << (r & 63);
+      *w64(T.X, r, g >> 6, T.rs, T.il) |= 1ull << (g & 63);
+      *w64(T.Z, r, r >> 6, T.rs, T.il) |= 1ull << (r & 63);
     } else {
-      T.X[(size_t)r * T.rs + n + r] = 1;
-      T.Z[(size_t)r * T.rs + r] = 1;
+      *b8(T.X, r, n + r, T.rs, T.il) = 1;
+      *b8(T.Z, r, r, T.rs, T.il) = 1;
     }
   }
   __syncthreads();
@@ -201,8 +204,8 @@ template <int D>
 SDIM_DEV uint32_t apply_odd(const DevOp& o, uint32_t xq, uint32_t zq, uint32_t xt, uint32_t zt,
                             const Tab& T, uint32_t w, uint32_t& dl, const Scr& S) {
   const uint32_t rs = T.rs;
-  uint32_t* pxq = row32(T.X, o.q, rs) + w;
-  uint32_t* pzq = row32(T.Z, o.q, rs) + w;
+  uint32_t* pxq = w32(T.X, o.q, w, rs, T.il);
+  uint32_t* pzq = w32(T.Z, o.q, w, rs, T.il);
   switch (op_kind(o)) {
     case OP_H:                                     // (x, z) <- (-z, x); phase += po (-z) x
       *pxq = negm<D>(zq); *pzq = xq;
@@ -225,11 +228,11 @@ SDIM_DEV uint32_t apply_odd(const DevOp& o, uint32_t xq, uint32_t zq, uint32_t x
       return 12;
     }
     case OP_SUM:                                   // x[t] += x[c]; z[c] += (d - 1) z[t]
-      *(row32(T.X, o.t, rs) + w) = addm<D>(xt, xq);
+      *w32(T.X, o.t, w, rs, T.il) = addm<D>(xt, xq);
       *pzq = subm<D>(zq, zt);
       return 24;
     case OP_SUMI:
-      *(row32(T.X, o.t, rs) + w) = subm<D>(xt, xq);
+      *w32(T.X, o.t, w, rs, T.il) = subm<D>(xt, xq);
       *pzq = addm<D>(zq, zt);
       return 24;
     case OP_PAULI:                                 // X^a Z^b: phase += b x - a z
@@ -248,8 +251,8 @@ SDIM_DEV uint32_t apply_odd(const DevOp& o, uint32_t xq, uint32_t zq, uint32_t x
 SDIM_DEV uint32_t apply_bin(const DevOp& o, uint64_t xq, uint64_t zq, uint64_t xt, uint64_t zt,
                             const Tab& T, uint32_t w, uint64_t& d0, uint64_t& d1, const Scr& S) {
   const uint32_t rs = T.rs;
-  uint64_t* pxq = row64(T.X, o.q, rs) + w;
-  uint64_t* pzq = row64(T.Z, o.q, rs) + w;
+  uint64_t* pxq = w64(T.X, o.q, w, rs, T.il);
+  uint64_t* pzq = w64(T.Z, o.q, w, rs, T.il);
   switch (op_kind(o)) {
     case OP_H:
     case OP_HI:                                    // swap; phase += 2 x z
@@ -266,7 +269,7 @@ SDIM_DEV uint32_t apply_bin(const DevOp& o, uint64_t xq, uint64_t zq, uint64_t x
       return 24;
     case OP_SUM:
     case OP_SUMI:
-      *(row64(T.X, o.t, rs) + w) = xt ^ xq;
+      *w64(T.X, o.t, w, rs, T.il) = xt ^ xq;
       *pzq = zq ^ zt;
       return 48;
     case OP_PAULI:                                 // phase += 2 (b x - a z)
@@ -329,7 +332,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
     tile = S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
     const int64_t left = (int64_t)(S.lds_end - tile);
     const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
-    uint64_t tw = room / (2ull * n);
+    uint64_t tw = room / (2ull * rows_padded(n));
     tw &= ~15ull;
     TW = tw >= 16 ? (uint32_t)(tw < rs ? tw : rs) : 0;
   }
@@ -339,13 +342,15 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
     uint32_t vw = rs;                            // bytes of each row in this view
     if (S.tiling && TW) {
       vw = (rs - c0 < TW) ? rs - c0 : TW;
-      V.X = tile; V.Z = tile + (size_t)n * vw; V.rs = vw; V.wofs = c0 / CPWb;
-      // load the tile (16-byte chunks)
-      const uint32_t cpr = vw / 16;
-      for (uint32_t i = tid; i < 2 * n * cpr; i += NT) {
-        const uint32_t rr = i / cpr, cc = i - rr * cpr;
-        const uint8_t* src = (rr < n ? T.X + (size_t)rr * rs : T.Z + (size_t)(rr - n) * rs) + c0 + 16 * cc;
-        *(uint4*)(tile + (size_t)rr * vw + 16 * cc) = *(const uint4*)src;
+      V.X = tile; V.Z = tile + (size_t)rows_padded(n) * vw; V.rs = vw; V.wofs = c0 / CPWb;
+      // load the tile: 128-byte pieces (8 rows x 16 bytes) of the X and Z blocks (tiling is
+      // used for 8-row interleaved tableaux only)
+      const uint32_t cpr = vw / 16, nrb = 2 * rows_padded(n) / 8;
+      for (uint32_t i = tid; i < nrb * cpr * 8; i += NT) {
+        const uint32_t ch = i >> 3, k = i & 7;
+        const uint32_t rb = ch / cpr, cc = ch - rb * cpr;
+        const uint8_t* src = T.X + ((size_t)rb * rs + c0 + 16 * cc) * 8 + 16 * k;
+        *(uint4*)(tile + ((size_t)rb * vw + 16 * cc) * 8 + 16 * k) = *(const uint4*)src;
         bytes += 16;
       }
       __syncthreads();
@@ -385,9 +390,9 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
               const uint32_t g = g0 + (uint32_t)b * NS;
               if (g < L) {
                 ob[b] = lops[g];
-                xq[b] = row64(V.X, ob[b].q, V.rs)[w];
-                zq[b] = row64(V.Z, ob[b].q, V.rs)[w];
-                if (two_rows(ob[b])) { xt[b] = row64(V.X, ob[b].t, V.rs)[w]; zt[b] = row64(V.Z, ob[b].t, V.rs)[w]; }
+                xq[b] = *w64(V.X, ob[b].q, w, V.rs, V.il);
+                zq[b] = *w64(V.Z, ob[b].q, w, V.rs, V.il);
+                if (two_rows(ob[b])) { xt[b] = *w64(V.X, ob[b].t, w, V.rs, V.il); zt[b] = *w64(V.Z, ob[b].t, w, V.rs, V.il); }
                 else { xt[b] = 0; zt[b] = 0; }
               }
             }
@@ -401,9 +406,9 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
               const uint32_t g = g0 + (uint32_t)b * NS;
               if (g < L) {
                 ob[b] = lops[g];
-                xq[b] = row32(V.X, ob[b].q, V.rs)[w];
-                zq[b] = row32(V.Z, ob[b].q, V.rs)[w];
-                if (two_rows(ob[b])) { xt[b] = row32(V.X, ob[b].t, V.rs)[w]; zt[b] = row32(V.Z, ob[b].t, V.rs)[w]; }
+                xq[b] = *w32(V.X, ob[b].q, w, V.rs, V.il);
+                zq[b] = *w32(V.Z, ob[b].q, w, V.rs, V.il);
+                if (two_rows(ob[b])) { xt[b] = *w32(V.X, ob[b].t, w, V.rs, V.il); zt[b] = *w32(V.Z, ob[b].t, w, V.rs, V.il); }
                 else { xt[b] = 0; zt[b] = 0; }
               }
             }
@@ -461,11 +466,12 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
 
     if (S.tiling && TW) {
       // write the tile back
-      const uint32_t cpr = vw / 16;
-      for (uint32_t i = tid; i < 2 * n * cpr; i += NT) {
-        const uint32_t rr = i / cpr, cc = i - rr * cpr;
-        uint8_t* dst = (rr < n ? T.X + (size_t)rr * rs : T.Z + (size_t)(rr - n) * rs) + c0 + 16 * cc;
-        *(uint4*)dst = *(const uint4*)(tile + (size_t)rr * vw + 16 * cc);
+      const uint32_t cpr = vw / 16, nrb = 2 * rows_padded(n) / 8;
+      for (uint32_t i = tid; i < nrb * cpr * 8; i += NT) {
+        const uint32_t ch = i >> 3, k = i & 7;
+        const uint32_t rb = ch / cpr, cc = ch - rb * cpr;
+        uint8_t* dst = T.X + ((size_t)rb * rs + c0 + 16 * cc) * 8 + 16 * k;
+        *(uint4*)dst = *(const uint4*)(tile + ((size_t)rb * vw + 16 * cc) * 8 + 16 * k);
         bytes += 16;
       }
       __syncthreads();
@@ -540,7 +546,7 @@ SDIM_DEV void rowq_groups(const Tab& V, uint32_t w0, uint32_t q, const Scr& S, i
   const uint32_t nsw = (n + CPW - 1) / CPW;
   const uint32_t ngr = nw / WPG;
   for (uint32_t g = tid; g < ngr; g += NT) {
-    const uint4 v4 = *(const uint4*)(V.X + (size_t)q * rs + 16 * g);
+    const uint4 v4 = *(const uint4*)b8(V.X, q, 16 * g, rs, V.il);
     if (rq) *(uint4*)(rq + 16 * g) = v4;
     uint4 f4;
     bool any;
@@ -609,8 +615,8 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
                 if (kk < nT) {
                   rr[b] = S.TR[kk];
                   sxv[b] = S.xp[rr[b]]; szv[b] = S.zp[rr[b]];
-                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
-                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
+                  zv[b] = *(const uint4*)b8(V.Z, rr[b], 16 * gi, rs, V.il);
+                  xv[b] = sxv[b] ? *(const uint4*)b8(V.X, rr[b], 16 * gi, rs, V.il) : make_uint4(0, 0, 0, 0);
                 }
               }
               #pragma unroll
@@ -622,13 +628,13 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
                     da0 ^= z0; da1 ^= z1;
                     uint4 o = xv[b];
                     o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
-                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = o;
+                    *(uint4*)b8(V.X, rr[b], 16 * gi, rs, V.il) = o;
                     bytes += 32;
                   }
                   if (szv[b]) {
                     uint4 o = zv[b];
                     o.x ^= f4.x; o.y ^= f4.y; o.z ^= f4.z; o.w ^= f4.w;
-                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) = o;
+                    *(uint4*)b8(V.Z, rr[b], 16 * gi, rs, V.il) = o;
                     bytes += 16;
                   }
                   bytes += 16;
@@ -652,8 +658,8 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
                   rr[b] = S.TR[kk];
                   sxv[b] = (e * S.xp[rr[b]]) % D;
                   szv[b] = (e * S.zp[rr[b]]) % D;
-                  zv[b] = *(const uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi);
-                  xv[b] = sxv[b] ? *(const uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) : make_uint4(0, 0, 0, 0);
+                  zv[b] = *(const uint4*)b8(V.Z, rr[b], 16 * gi, rs, V.il);
+                  xv[b] = sxv[b] ? *(const uint4*)b8(V.X, rr[b], 16 * gi, rs, V.il) : make_uint4(0, 0, 0, 0);
                 }
               }
               #pragma unroll
@@ -672,12 +678,12 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
                       dO[k] += (pz >> 8) & 0x00FF00FFu;
                       ow[k] = addm<D>(xw[k], mult<D>(t, fw[k]));
                     }
-                    *(uint4*)(V.X + (size_t)rr[b] * rs + 16 * gi) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+                    *(uint4*)b8(V.X, rr[b], 16 * gi, rs, V.il) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
                     bytes += 32;
                   }
                   if (szv[b]) {
                     const MulTab& t = S.mtab[szv[b]];
-                    *(uint4*)(V.Z + (size_t)rr[b] * rs + 16 * gi) =
+                    *(uint4*)b8(V.Z, rr[b], 16 * gi, rs, V.il) =
                         make_uint4(addm<D>(zw[0], mult<D>(t, fw[0])), addm<D>(zw[1], mult<D>(t, fw[1])),
                                    addm<D>(zw[2], mult<D>(t, fw[2])), addm<D>(zw[3], mult<D>(t, fw[3])));
                     bytes += 16;
@@ -787,12 +793,12 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       for (uint32_t r = tid; r < n; r += NT) {
         uint32_t xr, zr;
         if constexpr (BIN) {
-          xr = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-          zr = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+          xr = (uint32_t)((*w64(T.X, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
+          zr = (uint32_t)((*w64(T.Z, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
           bytes += 16;
         } else {
-          xr = T.X[(size_t)r * rs + p];
-          zr = T.Z[(size_t)r * rs + p];
+          xr = *b8(T.X, r, p, rs, T.il);
+          zr = *b8(T.Z, r, p, rs, T.il);
           bytes += 2;
         }
         S.xp[r] = (uint8_t)xr;
@@ -814,12 +820,12 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
           xs[b] = 0; zs[b] = 0;
           if (r < n) {
             if constexpr (BIN) {
-              xs[b] = (uint32_t)((row64(T.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-              zs[b] = (uint32_t)((row64(T.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+              xs[b] = (uint32_t)((*w64(T.X, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
+              zs[b] = (uint32_t)((*w64(T.Z, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
               bytes += 16;
             } else {
-              xs[b] = T.X[(size_t)r * rs + p];
-              zs[b] = T.Z[(size_t)r * rs + p];
+              xs[b] = *b8(T.X, r, p, rs, T.il);
+              zs[b] = *b8(T.Z, r, p, rs, T.il);
               bytes += 2;
             }
           }
@@ -886,28 +892,30 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     for (uint32_t r = tid; r < n; r += NT) {
       if constexpr (BIN) {
         const uint64_t xv = S.xp[r], zv = S.zp[r];
-        uint64_t* xr = row64(T.X, r, rs);
-        uint64_t* zr = row64(T.Z, r, rs);
         const uint32_t g = n + p;
+        uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
+        uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
         if ((p >> 6) == (g >> 6)) {
-          uint64_t x = xr[p >> 6], z = zr[p >> 6];
+          uint64_t x = *xp_, z = *zp_;
           x &= ~(1ull << (p & 63));
           x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
           z = (z & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
           z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
-          xr[p >> 6] = x; zr[p >> 6] = z;
+          *xp_ = x; *zp_ = z;
         } else {
-          xr[p >> 6] &= ~(1ull << (p & 63));
-          zr[p >> 6] = (zr[p >> 6] & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          xr[g >> 6] = (xr[g >> 6] & ~(1ull << (g & 63))) | (xv << (g & 63));
-          zr[g >> 6] = (zr[g >> 6] & ~(1ull << (g & 63))) | (zv << (g & 63));
+          uint64_t* xg_ = w64(T.X, r, g >> 6, rs, T.il);
+          uint64_t* zg_ = w64(T.Z, r, g >> 6, rs, T.il);
+          *xp_ &= ~(1ull << (p & 63));
+          *zp_ = (*zp_ & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
+          *xg_ = (*xg_ & ~(1ull << (g & 63))) | (xv << (g & 63));
+          *zg_ = (*zg_ & ~(1ull << (g & 63))) | (zv << (g & 63));
         }
         bytes += 32;
       } else {
-        T.X[(size_t)r * rs + p] = 0;
-        T.Z[(size_t)r * rs + p] = (uint8_t)(r == q);
-        T.X[(size_t)r * rs + n + p] = (uint8_t)((e * S.xp[r]) % D);
-        T.Z[(size_t)r * rs + n + p] = (uint8_t)((e * S.zp[r]) % D);
+        *b8(T.X, r, p, rs, T.il) = 0;
+        *b8(T.Z, r, p, rs, T.il) = (uint8_t)(r == q);
+        *b8(T.X, r, n + p, rs, T.il) = (uint8_t)((e * S.xp[r]) % D);
+        *b8(T.Z, r, n + p, rs, T.il) = (uint8_t)((e * S.zp[r]) % D);
         bytes += 4;
       }
     }
@@ -969,8 +977,6 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     uint32_t part = 0;
     for (uint32_t r = tid; r < n; r += NT) {
       if constexpr (BIN) {
-        const uint64_t* xr = row64(T.X, r, rs);
-        const uint64_t* zr = row64(T.Z, r, rs);
         uint64_t carry = 0, par = 0;
         for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
           uint64_t xa[DB], za[DB];
@@ -978,8 +984,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
           for (uint32_t u = 0; u < DB; u++) {
             const uint32_t w = w0 + u;
             const uint64_t f = w < nsw ? ((const uint64_t*)S.F)[w] : 0ull;
-            xa[u] = f ? xr[w] : 0ull;
-            za[u] = f ? zr[w] : 0ull;
+            xa[u] = f ? *w64(T.X, r, w, rs, T.il) : 0ull;
+            za[u] = f ? *w64(T.Z, r, w, rs, T.il) : 0ull;
           }
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
@@ -997,8 +1003,6 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         }
         part += (uint32_t)par;
       } else {
-        const uint32_t* xr = row32(T.X, r, rs);
-        const uint32_t* zr = row32(T.Z, r, rs);
         uint32_t U = 0, acc = 0;
         for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
           uint32_t xa[DB], za[DB];
@@ -1006,8 +1010,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
           for (uint32_t u = 0; u < DB; u++) {
             const uint32_t w = w0 + u;
             const uint32_t f = w < nsw ? ((const uint32_t*)S.F)[w] : 0u;
-            xa[u] = f ? xr[w] : 0u;
-            za[u] = f ? zr[w] : 0u;
+            xa[u] = f ? *w32(T.X, r, w, rs, T.il) : 0u;
+            za[u] = f ? *w32(T.Z, r, w, rs, T.il) : 0u;
           }
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
@@ -1069,13 +1073,13 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       const uint32_t m = S.shot[Sh + s];
       if constexpr (BIN) {
         if (m & 1) {
-          const uint64_t z = row64(T.Z, q, rs)[w];
+          const uint64_t z = *w64(T.Z, q, w, rs, T.il);
           if (z) { plane(T, s, 1)[w] ^= z; bytes += 24; }
         }
       } else {
         const uint32_t kk = (D - m) % D;
         if (kk) {
-          const uint32_t z = row32(T.Z, q, rs)[w];
+          const uint32_t z = *w32(T.Z, q, w, rs, T.il);
           if (z) {
             uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
             *ph = subm<D>(*ph, mult<D>(S.mtab[kk], z));
@@ -1101,10 +1105,10 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : lds;
   Scr S = carve<NT>(sbase, n, rs, Sh);
   S.lds_end = lds + P.lds_bytes;
-  S.tiling = !res;
+  S.tiling = !res && it.il == 3;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
+  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = tab + 2 * (size_t)rows_padded(n) * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
   unsigned long long bytes = 0, lbytes = 0;
   SDIM_TMARK(tItem);
 
@@ -1244,25 +1248,30 @@ SDIM_DEV bool gang_barrier(GangCtx& g, int* s_ok) {
   return *s_ok != 0;
 }
 
-// identity on the member's columns: V points at byte c_lo of every row, cb own bytes
+// identity on the member's columns: V covers bytes [c_lo, c_lo + cb) of every row
 template <int D, int NT>
 SDIM_DEV void gang_init_identity(const Tab& V, uint32_t c_lo, uint32_t cb) {
   const uint32_t n = V.n, rs = V.rs;
-  const uint32_t rows = 2 * n + (D == 2 ? 2 : 1) * V.S;   // X, Z and phase rows are consecutive
   const uint32_t q16 = cb / 16;
-  for (uint32_t i = threadIdx.x; i < rows * q16; i += NT) {
+  const uint32_t xzr = 2 * rows_padded(n);                  // rows of X then Z
+  for (uint32_t i = threadIdx.x; i < xzr * q16; i += NT) {
     const uint32_t r = i / q16, c = i - r * q16;
-    *(uint4*)(V.X + (size_t)r * rs + 16 * c) = make_uint4(0, 0, 0, 0);
+    *(uint4*)b8(V.X, r, 16 * c, rs, V.il) = make_uint4(0, 0, 0, 0);
+  }
+  const uint32_t prow = (D == 2 ? 2 : 1) * V.S;             // phase rows (row-major)
+  for (uint32_t i = threadIdx.x; i < prow * q16; i += NT) {
+    const uint32_t r = i / q16, c = i - r * q16;
+    *(uint4*)(V.Ph + (size_t)r * rs + 16 * c) = make_uint4(0, 0, 0, 0);
   }
   __syncthreads();
   for (uint32_t r = threadIdx.x; r < n; r += NT) {
     if constexpr (D == 2) {
       const uint32_t g = n + r;
-      if (g >= c_lo * 8 && g < (c_lo + cb) * 8) row64(V.X, r, rs)[(g >> 6) - c_lo / 8] |= 1ull << (g & 63);
-      if (r >= c_lo * 8 && r < (c_lo + cb) * 8) row64(V.Z, r, rs)[(r >> 6) - c_lo / 8] |= 1ull << (r & 63);
+      if (g >= c_lo * 8 && g < (c_lo + cb) * 8) *w64(V.X, r, (g >> 6) - c_lo / 8, rs, V.il) |= 1ull << (g & 63);
+      if (r >= c_lo * 8 && r < (c_lo + cb) * 8) *w64(V.Z, r, (r >> 6) - c_lo / 8, rs, V.il) |= 1ull << (r & 63);
     } else {
-      if (n + r >= c_lo && n + r < c_lo + cb) V.X[(size_t)r * rs + n + r - c_lo] = 1;
-      if (r >= c_lo && r < c_lo + cb) V.Z[(size_t)r * rs + r - c_lo] = 1;
+      if (n + r >= c_lo && n + r < c_lo + cb) *b8(V.X, r, n + r - c_lo, rs, V.il) = 1;
+      if (r >= c_lo && r < c_lo + cb) *b8(V.Z, r, r - c_lo, rs, V.il) = 1;
     }
   }
   __syncthreads();
@@ -1353,11 +1362,11 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       for (uint32_t r = r0 + tid; r < r1; r += NT) {
         uint32_t xr, zr;
         if constexpr (BIN) {
-          xr = (uint32_t)((row64(Tg.X, r, rs)[p >> 6] >> (p & 63)) & 1);
-          zr = (uint32_t)((row64(Tg.Z, r, rs)[p >> 6] >> (p & 63)) & 1);
+          xr = (uint32_t)((*w64(Tg.X, r, p >> 6, rs, Tg.il) >> (p & 63)) & 1);
+          zr = (uint32_t)((*w64(Tg.Z, r, p >> 6, rs, Tg.il) >> (p & 63)) & 1);
         } else {
-          xr = Tg.X[(size_t)r * rs + p];
-          zr = Tg.Z[(size_t)r * rs + p];
+          xr = *b8(Tg.X, r, p, rs, Tg.il);
+          zr = *b8(Tg.Z, r, p, rs, Tg.il);
         }
         stg[r] = (uint8_t)(xr | (zr << 4));
       }
@@ -1461,35 +1470,35 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       for (uint32_t r = tid; r < n; r += NT) {
         if constexpr (BIN) {
           const uint64_t xv = S.xp[r], zv = S.zp[r];
-          uint64_t* xr = row64(V.X, r, rs);
-          uint64_t* zr = row64(V.Z, r, rs);
           const uint32_t g = n + p;
           if (own_p) {
-            const uint32_t wl = (p >> 6) - w0;
-            uint64_t x = xr[wl] & ~(1ull << (p & 63));
-            uint64_t z = (zr[wl] & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
+            uint64_t* xw = w64(V.X, r, (p >> 6) - w0, rs, V.il);
+            uint64_t* zw = w64(V.Z, r, (p >> 6) - w0, rs, V.il);
+            uint64_t x = *xw & ~(1ull << (p & 63));
+            uint64_t z = (*zw & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
             if ((g >> 6) == (p >> 6)) {
               x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
               z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
             }
-            xr[wl] = x; zr[wl] = z;
+            *xw = x; *zw = z;
           }
           if (own_g && (g >> 6) != (p >> 6)) {
-            const uint32_t wl = (g >> 6) - w0;
-            xr[wl] = (xr[wl] & ~(1ull << (g & 63))) | (xv << (g & 63));
-            zr[wl] = (zr[wl] & ~(1ull << (g & 63))) | (zv << (g & 63));
+            uint64_t* xw = w64(V.X, r, (g >> 6) - w0, rs, V.il);
+            uint64_t* zw = w64(V.Z, r, (g >> 6) - w0, rs, V.il);
+            *xw = (*xw & ~(1ull << (g & 63))) | (xv << (g & 63));
+            *zw = (*zw & ~(1ull << (g & 63))) | (zv << (g & 63));
           }
           if (own_p) bytes += 16;
           if (own_g && (g >> 6) != (p >> 6)) bytes += 16;
         } else {
           if (own_p) {
-            V.X[(size_t)r * rs + p - c_lo] = 0;
-            V.Z[(size_t)r * rs + p - c_lo] = (uint8_t)(r == q);
+            *b8(V.X, r, p - c_lo, rs, V.il) = 0;
+            *b8(V.Z, r, p - c_lo, rs, V.il) = (uint8_t)(r == q);
             bytes += 2;
           }
           if (own_g) {
-            V.X[(size_t)r * rs + n + p - c_lo] = (uint8_t)((e * S.xp[r]) % D);
-            V.Z[(size_t)r * rs + n + p - c_lo] = (uint8_t)((e * S.zp[r]) % D);
+            *b8(V.X, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.xp[r]) % D);
+            *b8(V.Z, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.zp[r]) % D);
             bytes += 2;
           }
         }
@@ -1534,10 +1543,9 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     for (uint32_t w = tid; w < ns_own; w += NT) {
       const uint32_t gw = w0 + w;
       if constexpr (BIN) {
-        const uint64_t* rq = row64(Tg.X, q, rs);
         const uint32_t g = n + 64 * gw;
-        const uint64_t lo = rq[g >> 6] >> (g & 63);
-        const uint64_t hi = ((g & 63) && (g >> 6) + 1 < rs / 8) ? rq[(g >> 6) + 1] << (64 - (g & 63)) : 0ull;
+        const uint64_t lo = *w64(Tg.X, q, g >> 6, rs, Tg.il) >> (g & 63);
+        const uint64_t hi = ((g & 63) && (g >> 6) + 1 < rs / 8) ? *w64(Tg.X, q, (g >> 6) + 1, rs, Tg.il) << (64 - (g & 63)) : 0ull;
         uint64_t f = lo | hi;
         const uint32_t lim = n - 64 * gw;
         if (lim < 64) f &= (1ull << lim) - 1;
@@ -1548,7 +1556,7 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         #pragma unroll
         for (int b = 0; b < 4; b++) {
           const uint32_t c = 4 * gw + b;
-          if (c < n) f |= (uint32_t)Tg.X[(size_t)q * rs + n + c] << (8 * b);
+          if (c < n) f |= (uint32_t)*b8(Tg.X, q, n + c, rs, Tg.il) << (8 * b);
         }
         ((uint32_t*)S.F)[w] = f;
         bytes += 4;
@@ -1560,13 +1568,11 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     for (uint32_t r = tid; r < n; r += NT) {
       uint32_t sx = 0, sz = 0;
       if constexpr (BIN) {
-        const uint64_t* xr = row64(V.X, r, rs);
-        const uint64_t* zr = row64(V.Z, r, rs);
         uint64_t carry = 0, par = 0, pa = 0;
         for (uint32_t w = 0; w < ns_own; w++) {
           const uint64_t f = ((const uint64_t*)S.F)[w];
           if (!f) continue;
-          const uint64_t a = xr[w] & f, b = zr[w] & f;
+          const uint64_t a = *w64(V.X, r, w, rs, V.il) & f, b = *w64(V.Z, r, w, rs, V.il) & f;
           uint64_t y = b;
           y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
           const uint64_t pre = (y << 1) ^ (carry ? ~0ull : 0ull);
@@ -1578,13 +1584,11 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
         part += (uint32_t)par;
         sx = (uint32_t)pa; sz = (uint32_t)carry;
       } else {
-        const uint32_t* xr = row32(V.X, r, rs);
-        const uint32_t* zr = row32(V.Z, r, rs);
         uint32_t U = 0, acc = 0, ax = 0;
         for (uint32_t w = 0; w < ns_own; w++) {
           const uint32_t f = ((const uint32_t*)S.F)[w];
           if (!f) continue;
-          const uint32_t x = xr[w], z = zr[w];
+          const uint32_t x = *w32(V.X, r, w, rs, V.il), z = *w32(V.Z, r, w, rs, V.il);
           #pragma unroll
           for (int b = 0; b < 4; b++) {
             const uint32_t fb = byte_of(f, b);
@@ -1665,13 +1669,13 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
       const uint32_t m = S.shot[Sh + s];
       if constexpr (BIN) {
         if (m & 1) {
-          const uint64_t z = row64(V.Z, q, rs)[w];
+          const uint64_t z = *w64(V.Z, q, w, rs, V.il);
           if (z) { plane(V, s, 1)[w] ^= z; bytes += 24; }
         }
       } else {
         const uint32_t kk = (D - m) % D;
         if (kk) {
-          const uint32_t z = row32(V.Z, q, rs)[w];
+          const uint32_t z = *w32(V.Z, q, w, rs, V.il);
           if (z) {
             uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
             *ph = subm<D>(*ph, mult<D>(S.mtab[kk], z));
@@ -1696,10 +1700,12 @@ SDIM_DEV bool gang_item(const ItemDesc& it, const KParams& P, GangCtx& Gc, uint8
   const uint32_t w1 = (w0 + cwn < NWall) ? w0 + cwn : NWall;
   uint8_t* tab = P.arena + it.tab_off;
   Tab Tg;
-  Tg.X = tab; Tg.Z = tab + (size_t)n * rs; Tg.Ph = tab + 2 * (size_t)n * rs;
-  Tg.n = n; Tg.rs = rs; Tg.S = Sh; Tg.wofs = 0; Tg.rs_full = rs; Tg.nw = NWall;
+  Tg.X = tab; Tg.Z = tab + (size_t)rows_padded(n) * rs; Tg.Ph = tab + 2 * (size_t)rows_padded(n) * rs;
+  Tg.n = n; Tg.rs = rs; Tg.S = Sh; Tg.wofs = 0; Tg.rs_full = rs; Tg.nw = NWall; Tg.il = it.il;
   Tab V = Tg;
-  V.X += (size_t)w0 * CPWb; V.Z += (size_t)w0 * CPWb; V.Ph += (size_t)w0 * CPWb;
+  // column bytes [c, c + k) of every row start at byte c << il of an X/Z block (c a
+  // multiple of 16); the phase vectors are row-major
+  V.X += ((size_t)w0 * CPWb) << Tg.il; V.Z += ((size_t)w0 * CPWb) << Tg.il; V.Ph += (size_t)w0 * CPWb;
   V.rs_full = cwn * CPWb; V.nw = w1 - w0;
   Scr S = carve<NT>(lds, n, cwn * CPWb, Sh);
   S.lds_end = lds + P.lds_bytes;
@@ -1793,8 +1799,8 @@ __global__ __launch_bounds__(256) void plain_init_kernel(KParams P) {
   const ItemDesc it = P.items[blockIdx.x];
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)it.n * it.rs; T.Ph = tab + 2 * (size_t)it.n * it.rs;
-  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs; T.nw = it.rs / (D == 2 ? 8 : 4);
+  T.X = tab; T.Z = tab + (size_t)rows_padded(it.n) * it.rs; T.Ph = tab + 2 * (size_t)rows_padded(it.n) * it.rs;
+  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs; T.nw = it.rs / (D == 2 ? 8 : 4); T.il = it.il;
   init_identity<D, 256>(T, tableau_bytes(it.n, D, it.S));
 }
 
@@ -1811,8 +1817,8 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
   if (w >= NWd) return;
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
+  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = tab + 2 * (size_t)rows_padded(n) * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
   Scr S;
   S.mtab = mt;
   S.tiling = false;
@@ -1827,10 +1833,10 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
       if (!ab) continue;
       const uint32_t a = ab & 15, b = (ab >> 4) & 15;
       if constexpr (D == 2) {
-        const uint64_t x = row64(T.X, q, rs)[w], z = row64(T.Z, q, rs)[w];
+        const uint64_t x = *w64(T.X, q, w, rs, T.il), z = *w64(T.Z, q, w, rs, T.il);
         plane(T, s, 1)[w] ^= ((b & 1) ? x : 0ull) ^ ((a & 1) ? z : 0ull);
       } else {
-        const uint32_t x = row32(T.X, q, rs)[w], z = row32(T.Z, q, rs)[w];
+        const uint32_t x = *w32(T.X, q, w, rs, T.il), z = *w32(T.Z, q, w, rs, T.il);
         uint32_t* ph = (uint32_t*)phase_vec(T, s) + w;
         *ph = addm<D>(*ph, subm<D>(mult<D>(mt[b], x), mult<D>(mt[a], z)));
       }
@@ -1840,8 +1846,8 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
   const bool two = two_rows(o);
   if constexpr (D == 2) {
     uint64_t d0 = 0, d1 = 0;
-    const uint64_t xq = row64(T.X, o.q, rs)[w], zq = row64(T.Z, o.q, rs)[w];
-    const uint64_t xt = two ? row64(T.X, o.t, rs)[w] : 0ull, zt = two ? row64(T.Z, o.t, rs)[w] : 0ull;
+    const uint64_t xq = *w64(T.X, o.q, w, rs, T.il), zq = *w64(T.Z, o.q, w, rs, T.il);
+    const uint64_t xt = two ? *w64(T.X, o.t, w, rs, T.il) : 0ull, zt = two ? *w64(T.Z, o.t, w, rs, T.il) : 0ull;
     apply_bin(o, xq, zq, xt, zt, T, w, d0, d1, S);
     if (d0 | d1) {
       for (uint32_t s = 0; s < Sh; s++) {
@@ -1854,8 +1860,8 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
     }
   } else {
     uint32_t dl = 0;
-    const uint32_t xq = row32(T.X, o.q, rs)[w], zq = row32(T.Z, o.q, rs)[w];
-    const uint32_t xt = two ? row32(T.X, o.t, rs)[w] : 0u, zt = two ? row32(T.Z, o.t, rs)[w] : 0u;
+    const uint32_t xq = *w32(T.X, o.q, w, rs, T.il), zq = *w32(T.Z, o.q, w, rs, T.il);
+    const uint32_t xt = two ? *w32(T.X, o.t, w, rs, T.il) : 0u, zt = two ? *w32(T.Z, o.t, w, rs, T.il) : 0u;
     apply_odd<D>(o, xq, zq, xt, zt, T, w, dl, S);
     if (dl) {
       for (uint32_t s = 0; s < Sh; s++) {
@@ -1873,8 +1879,8 @@ __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
   const uint32_t n = it.n, rs = it.rs, Sh = it.S;
   uint8_t* tab = P.arena + it.tab_off;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)n * rs; T.Ph = tab + 2 * (size_t)n * rs; T.n = n; T.rs = rs; T.S = Sh;
-  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4);
+  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = tab + 2 * (size_t)rows_padded(n) * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
   const Scr S = carve<NT>(lds, n, rs, Sh);
   if (threadIdx.x < 16) S.mtab[threadIdx.x] = make_tab<D>(threadIdx.x < D ? threadIdx.x : 0);
   if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;

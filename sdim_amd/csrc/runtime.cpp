@@ -353,12 +353,12 @@ void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max,
 uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
 
 // ---------------------------------------------------------------- tableau upload / download
-void pack_tableau(const JobPlan& jp, uint32_t S, const sdim_job& j, std::vector<uint8_t>& buf) {
+void pack_tableau(const JobPlan& jp, uint32_t S, uint32_t il, const sdim_job& j, std::vector<uint8_t>& buf) {
   const uint32_t n = (uint32_t)jp.n, d = (uint32_t)jp.d, rs = row_stride(n, d);
   buf.assign(tableau_bytes(n, d, S), 0);
-  uint8_t* X = buf.data();
-  uint8_t* Z = X + (size_t)n * rs;
-  uint8_t* Ph = Z + (size_t)n * rs;
+  uint8_t* X = buf.data();                                     // X then Z: one interleaved block
+  uint8_t* Ph = X + 2 * (size_t)rows_padded(n) * rs;            // phase vectors, row-major
+  const uint32_t np = rows_padded(n);
   const int64_t *x = j.init[0], *z = j.init[1], *ph = j.init[2], *dx = j.init[3], *dz = j.init[4], *dph = j.init[5];
   const int64_t ord = (d == 2) ? 4 : d;
   auto md = [&](int64_t v, int64_t m) { int64_t r = v % m; return r < 0 ? r + m : r; };
@@ -367,11 +367,11 @@ void pack_tableau(const JobPlan& jp, uint32_t S, const sdim_job& j, std::vector<
       const int64_t xv = md(g < n ? x[(size_t)r * n + g] : dx[(size_t)r * n + g - n], d);
       const int64_t zv = md(g < n ? z[(size_t)r * n + g] : dz[(size_t)r * n + g - n], d);
       if (d == 2) {
-        if (xv) ((uint64_t*)(X + (size_t)r * rs))[g >> 6] |= 1ull << (g & 63);
-        if (zv) ((uint64_t*)(Z + (size_t)r * rs))[g >> 6] |= 1ull << (g & 63);
+        if (xv) *(uint64_t*)(X + cell(r, 8 * (g >> 6), rs, il)) |= 1ull << (g & 63);
+        if (zv) *(uint64_t*)(X + cell(np + r, 8 * (g >> 6), rs, il)) |= 1ull << (g & 63);
       } else {
-        X[(size_t)r * rs + g] = (uint8_t)xv;
-        Z[(size_t)r * rs + g] = (uint8_t)zv;
+        X[cell(r, g, rs, il)] = (uint8_t)xv;
+        X[cell(np + r, g, rs, il)] = (uint8_t)zv;
       }
     }
   }
@@ -388,21 +388,20 @@ void pack_tableau(const JobPlan& jp, uint32_t S, const sdim_job& j, std::vector<
   }
 }
 
-void unpack_tableau(const uint8_t* tab, uint32_t n, uint32_t d, uint32_t s, int64_t* x, int64_t* z, int64_t* ph,
-                    int64_t* dx, int64_t* dz, int64_t* dph) {
-  const uint32_t rs = row_stride(n, d);
-  const uint8_t* X = tab;
-  const uint8_t* Z = X + (size_t)n * rs;
-  const uint8_t* Ph = Z + (size_t)n * rs;
+void unpack_tableau(const uint8_t* tab, uint32_t n, uint32_t d, uint32_t il, uint32_t s, int64_t* x, int64_t* z,
+                    int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) {
+  const uint32_t rs = row_stride(n, d), np = rows_padded(n);
+  const uint8_t* X = tab;                                      // X then Z: one interleaved block
+  const uint8_t* Ph = X + 2 * (size_t)np * rs;
   for (uint32_t r = 0; r < n; r++) {
     for (uint32_t g = 0; g < 2 * n; g++) {
       int64_t xv, zv;
       if (d == 2) {
-        xv = (((const uint64_t*)(X + (size_t)r * rs))[g >> 6] >> (g & 63)) & 1;
-        zv = (((const uint64_t*)(Z + (size_t)r * rs))[g >> 6] >> (g & 63)) & 1;
+        xv = (*(const uint64_t*)(X + cell(r, 8 * (g >> 6), rs, il)) >> (g & 63)) & 1;
+        zv = (*(const uint64_t*)(X + cell(np + r, 8 * (g >> 6), rs, il)) >> (g & 63)) & 1;
       } else {
-        xv = X[(size_t)r * rs + g];
-        zv = Z[(size_t)r * rs + g];
+        xv = X[cell(r, g, rs, il)];
+        zv = X[cell(np + r, g, rs, il)];
       }
       if (g < n) { x[(size_t)r * n + g] = xv; z[(size_t)r * n + g] = zv; }
       else { dx[(size_t)r * n + g - n] = xv; dz[(size_t)r * n + g - n] = zv; }
@@ -784,7 +783,10 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     jp.first_item = (uint32_t)items.size();
     std::vector<uint8_t> init_buf;
     const bool given = j.init[0] && j.init[1] && j.init[2] && j.init[3] && j.init[4] && j.init[5];
-    if (given) pack_tableau(jp, S, j, init_buf);
+    // 8-row interleave for tableaux swept by one workgroup; gang members and the plain path
+    // apply gates straight from HBM and keep rows contiguous
+    const uint32_t il = (mode == SDIM_MODE_BATCHED && !gang_cb) ? 3u : 0u;
+    if (given) pack_tableau(jp, S, il, j, init_buf);
     for (int64_t s0 = 0; s0 < j.shots; s0 += S) {
       ItemDesc it;
       memset(&it, 0, sizeof(it));
@@ -800,6 +802,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       it.rs = rs;
       it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0);
       it.n_meas = (uint32_t)jp.n_meas;
+      it.il = il;
       it.rec_off = rec_total;
       rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
       if (given) inits.emplace_back((uint32_t)items.size(), init_buf);
@@ -1064,7 +1067,7 @@ int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, 
     if (shot >= it.shot0 && shot < (int64_t)it.shot0 + it.S) {
       std::vector<uint8_t> buf(tableau_bytes(it.n, it.d, it.S));
       HIP_OK(hipMemcpy(buf.data(), (uint8_t*)t->arena.p + it.tab_off, buf.size(), hipMemcpyDeviceToHost));
-      unpack_tableau(buf.data(), it.n, it.d, (uint32_t)(shot - it.shot0), x, z, ph, dx, dz, dph);
+      unpack_tableau(buf.data(), it.n, it.d, it.il, (uint32_t)(shot - it.shot0), x, z, ph, dx, dz, dph);
       return 0;
     }
   }

@@ -9,7 +9,15 @@
 //   d = 2 : X[n][rs], Z[n][rs] bit-packed (generator g = bit g%64 of u64 word
 //           g/64), Ph[S][2][rs] two bit-planes p0,p1 with phase = p0 + 2 p1
 //           (mod 4);  rs = 8 * round_up(ceil(G/64), 2) bytes.
-//   Bytes of an item: T = 2 n rs + S rs (odd) or 2 n rs + 2 S rs (d = 2).
+//   The X and Z blocks are qudit-major, rows padded to np = round_up(n, 8), with
+//   an optional 8-row interleave (il = 3; il = 0 is plain row-major): byte b of
+//   row r sits at cell(r, b, rs, il) = ((r >> il) rs + 16 (b/16)) 2^il +
+//   16 (r mod 2^il) + b%16.  With il = 3 a 128-byte line holds 16 bytes of 8
+//   consecutive qudits, so reading one generator column (the pivot of a
+//   measurement) touches n/8 lines instead of n; items whose gates run straight
+//   from HBM (gang members, the plain path) keep il = 0.  Z starts np rows after
+//   X (Z rows are X rows np.. of one block); the phase vectors follow, row-major.
+//   Bytes of an item: T = 2 np rs + S rs (odd) or 2 np rs + 2 S rs (d = 2).
 // The X/Z blocks are identical for every shot of a circuit (only the phases
 // depend on the random outcomes and noise draws), so an item stores them once
 // and keeps one phase vector per shot.
@@ -62,7 +70,7 @@ struct ItemDesc {
   uint32_t rs;         // row stride in bytes
   uint32_t flags;
   uint32_t n_meas;
-  uint32_t pad;
+  uint32_t il;         // X/Z row interleave: 3 (8-row blocks) or 0 (row-major)
 };
 
 static inline __host__ __device__ uint32_t row_stride(uint32_t n, uint32_t d) {
@@ -75,9 +83,16 @@ static inline __host__ __device__ uint32_t row_stride(uint32_t n, uint32_t d) {
   return (G + 15) & ~15u;
 }
 
+static inline __host__ __device__ uint32_t rows_padded(uint32_t n) { return (n + 7u) & ~7u; }
+
+// byte offset of byte b of row r in an X/Z block of row stride rs with row interleave il
+static inline __host__ __device__ uint64_t cell(uint32_t r, uint32_t b, uint32_t rs, uint32_t il) {
+  return (((uint64_t)(r >> il) * rs + (b & ~15u)) << il) + (r & ((1u << il) - 1u)) * 16u + (b & 15u);
+}
+
 static inline __host__ __device__ uint64_t tableau_bytes(uint32_t n, uint32_t d, uint32_t S) {
   uint64_t rs = row_stride(n, d);
-  return 2ull * n * rs + (d == 2 ? 2ull : 1ull) * S * rs;
+  return 2ull * rows_padded(n) * rs + (d == 2 ? 2ull : 1ull) * S * rs;
 }
 
 }  // namespace sdim
