@@ -170,11 +170,12 @@ SDIM_DEV unsigned long long block_sum64(unsigned long long v, uint32_t* red) {
 
 // ------------------------------------------------------------------ tableau init / copy
 template <int D, int NT>
-SDIM_DEV void init_identity(const Tab& T, uint64_t total) {
-  // zero everything, then the diagonals: stabilizer i = Z_i, destabilizer i = X_i
+SDIM_DEV void init_identity(const Tab& T, uint64_t xz_bytes, uint64_t ph_bytes) {
+  // zero X, Z and the phase vectors, then the diagonals: stabilizer i = Z_i, destabilizer i = X_i
   uint4* p = (uint4*)T.X;
-  const uint64_t nv = total / 16;
-  for (uint64_t i = threadIdx.x; i < nv; i += NT) p[i] = make_uint4(0, 0, 0, 0);
+  for (uint64_t i = threadIdx.x; i < xz_bytes / 16; i += NT) p[i] = make_uint4(0, 0, 0, 0);
+  uint4* q = (uint4*)T.Ph;
+  for (uint64_t i = threadIdx.x; i < ph_bytes / 16; i += NT) q[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   const uint32_t n = T.n;
   for (uint32_t r = threadIdx.x; r < n; r += NT) {
@@ -1098,16 +1099,18 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 template <int D, int NT>
 SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   const bool res = (it.flags & ITEM_LDS) != 0;
+  const bool phl = !res && (it.flags & ITEM_PH_LDS) != 0;     // phases kept in LDS
   const uint32_t n = it.n, rs = it.rs, Sh = it.S;
   const uint64_t T_bytes = tableau_bytes(n, D, Sh);
+  const uint64_t XZ_bytes = 2ull * rows_padded(n) * rs, PH_bytes = T_bytes - XZ_bytes;
   uint8_t* gtab = P.arena + it.tab_off;
   uint8_t* tab = res ? lds : gtab;
-  uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : lds;
+  uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : (phl ? lds + ((PH_bytes + 15) & ~15ull) : lds);
   Scr S = carve<NT>(sbase, n, rs, Sh);
   S.lds_end = lds + P.lds_bytes;
   S.tiling = !res && it.il == 3;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = tab + 2 * (size_t)rows_padded(n) * rs; T.n = n; T.rs = rs; T.S = Sh;
+  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = phl ? lds : tab + XZ_bytes; T.n = n; T.rs = rs; T.S = Sh;
   T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
   unsigned long long bytes = 0, lbytes = 0;
   SDIM_TMARK(tItem);
@@ -1116,9 +1119,10 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
   if (it.flags & ITEM_INIT_GIVEN) {
     if (res) copy16<NT>(lds, gtab, T_bytes);
+    if (phl) copy16<NT>(lds, gtab + XZ_bytes, PH_bytes);
     __syncthreads();
   } else {
-    init_identity<D, NT>(T, T_bytes);
+    init_identity<D, NT>(T, XZ_bytes, PH_bytes);
   }
   if (threadIdx.x == 0) bytes += T_bytes;
   SDIM_TADD(P, 9, tItem);
@@ -1142,6 +1146,10 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   if (res) {
     copy16<NT>(gtab, lds, T_bytes);
     if (threadIdx.x == 0) bytes += T_bytes;
+  }
+  if (phl) {
+    copy16<NT>(gtab + XZ_bytes, lds, PH_bytes);
+    if (threadIdx.x == 0) bytes += PH_bytes;
   }
   SDIM_TADD(P, 10, tOut);
   SDIM_TADD(P, 11, tItem);
@@ -1801,7 +1809,8 @@ __global__ __launch_bounds__(256) void plain_init_kernel(KParams P) {
   Tab T;
   T.X = tab; T.Z = tab + (size_t)rows_padded(it.n) * it.rs; T.Ph = tab + 2 * (size_t)rows_padded(it.n) * it.rs;
   T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs; T.nw = it.rs / (D == 2 ? 8 : 4); T.il = it.il;
-  init_identity<D, 256>(T, tableau_bytes(it.n, D, it.S));
+  const uint64_t xz = 2ull * rows_padded(it.n) * it.rs;
+  init_identity<D, 256>(T, xz, tableau_bytes(it.n, D, it.S) - xz);
 }
 
 template <int D>

@@ -763,12 +763,22 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     const bool resident = cls >= 0 && kClasses[cls].resident;
     jp.S_item = S;
     jp.resident = resident;
+    // one-workgroup HBM items keep the shots' phase vectors in LDS when they take at most
+    // a quarter of the class budget (the rest stays for scratch, noise tables and tiles)
+    bool ph_lds = false;
+    if (mode == SDIM_MODE_BATCHED && !resident && !gang_cb) {
+      const uint64_t ph = tableau_bytes(n, d, S) - 2ull * rows_padded(n) * rs;
+      const uint64_t budget = std::min<uint32_t>(kClasses[cls].lds, cap);
+      ph_lds = ph <= budget / 4 &&
+               ((ph + 15) & ~15ull) + scratch_bytes(n, rs, S, nt_of(c, kClasses[cls])) + (has_noise ? noise_bytes(1, rs, S) : 0) <= budget;
+    }
     // noise ops per segment that fit next to the tableau / scratch
     if (mode == SDIM_MODE_BATCHED) {
       const LaunchClass& L = kClasses[cls];
       const uint32_t w = gang_cb ? gang_cb : rs;     // width of the per-column LDS arrays
       const uint32_t nt = gang_cb ? (uint32_t)L.nt : nt_of(c, L);
-      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, w, S, nt);
+      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, w, S, nt) +
+                            (ph_lds ? ((tableau_bytes(n, d, S) - 2ull * rows_padded(n) * rs + 15) & ~15ull) : 0);
       const uint64_t budget = std::min<uint32_t>(L.lds, cap);
       // HBM-resident one-workgroup items keep half of the remaining LDS for the column tiles of gate segments
       const uint64_t room = budget > base ? ((resident || gang_cb) ? budget - base : (budget - base) / 2) : 0;
@@ -800,7 +810,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       it.S = (uint32_t)std::min<int64_t>(S, j.shots - s0);
       it.shot0 = (uint32_t)s0;
       it.rs = rs;
-      it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0);
+      it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0) | (ph_lds ? ITEM_PH_LDS : 0);
       it.n_meas = (uint32_t)jp.n_meas;
       it.il = il;
       it.rec_off = rec_total;

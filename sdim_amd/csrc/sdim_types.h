@@ -53,6 +53,7 @@ static inline __host__ __device__ uint32_t op_noise_slot(const DevOp& o) { retur
 enum ItemFlags : uint32_t {
   ITEM_LDS = 1u,        // tableau resident in LDS for the whole item
   ITEM_INIT_GIVEN = 2u, // start from the tableau already stored at tab_off (else identity)
+  ITEM_PH_LDS = 4u,     // X/Z in HBM, the shots' phase vectors in LDS for the whole item
 };
 
 struct ItemDesc {
