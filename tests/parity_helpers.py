@@ -39,7 +39,7 @@ def oracle_of(job, keep_shots):
 def check_job(tk, ji, job, tableaux=True, shots=None):
     """Records of every shot of job `ji` of ticket `tk`, and the final tableau of the
     shots in `shots` (all by default), bit-exact against the C oracle."""
-    keep = job.shots if tableaux else 0
+    keep = (job.shots if shots is None else max(shots) + 1) if tableaux else 0
     rec_o, tabs_o = oracle_of(job, keep)
     rec = tk.records(ji)                      # [M, shots]
     got = np.stack([rec['qudit_index'], rec['meas_round'], rec['shot'],

@@ -99,3 +99,35 @@ def test_uneven_batch_workload_sample():
         check_job(tk, i, wl.jobs[i], tableaux=True, shots=(0, 7))
     tk.release()
     eng.close()
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_plain_small_configuration(mode):
+    """d=2, n=24, 500 uniform {H, P, CNOT} then measure-all, 1024 shots: the records of every
+    shot and the final tableaux of four shots against the oracle."""
+    from sdim_amd import workloads as W
+    from sdim_amd.engine import Engine
+    wl = W.plain_small()
+    eng = Engine()
+    print(f"[parity-config] plain_small mode={mode}", flush=True)
+    tk = eng.run(wl.jobs, mode=mode)
+    check_job(tk, 0, wl.jobs[0], tableaux=True, shots=(0, 1, 511, 1023))
+    tk.release()
+    eng.close()
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_repetition_configuration(mode):
+    """d=3 repetition ring (17 data + 17 ancilla, 10 rounds of SUMs, ancilla M and RESET, X^a /
+    Z^b errors with p=1e-3 after every SUM), 4096 shots: every record against the oracle."""
+    from sdim_amd import workloads as W
+    from sdim_amd.engine import Engine
+    wl = W.repetition_code()
+    eng = Engine()
+    print(f"[parity-config] repetition mode={mode}", flush=True)
+    tk = eng.run(wl.jobs, mode=mode)
+    check_job(tk, 0, wl.jobs[0], tableaux=True, shots=(0, 4095))
+    rec = tk.records(0)
+    assert rec.shape == (340, 4096)
+    tk.release()
+    eng.close()
