@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on the workload first and measure with its choice")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="requests/s (0: 60%% of a measured burst rate)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -197,6 +199,11 @@ def main():
         opts["shots_per_item"] = args.shots_per_item
     if args.threads:
         opts["threads"] = args.threads
+    tuned = None
+    if args.tune:
+        from sdim_amd import autotune
+        tuned = autotune.tune(jobs, reps=max(1, min(args.steps, 3)), max_seconds=150.0)
+        opts.update(tuned["options"])
     eng = Engine(**opts)
 
     t_submit = time.perf_counter()
@@ -241,7 +248,9 @@ def main():
         "vs_baseline": None, "dtype": "u8,u64", "data": "synthetic",
         "config": {"workload": wl.name, "description": wl.description, "circuits": len(jobs),
                    "gate_shots": gate_shots, "items": st["items"], "lds_items": st["lds_items"],
-                   "threads": eng.get_option("threads"), "shots_per_item": eng.get_option("shots_per_item")},
+                   "threads": eng.get_option("threads"), "shots_per_item": eng.get_option("shots_per_item"),
+                   "options": ({k: eng.get_option(k) for k in tuned["options"]} if tuned else "defaults"),
+                   "tuned_from_ms": tuned["default_ms"] if tuned else None},
         "device_ms_per_step": dev_ms / args.steps,
         "first_submit_s": t_submit,
         "roofline": roof("hbm", HBM_PEAK_GBS) or roof("lds", LDS_PEAK_GBS),

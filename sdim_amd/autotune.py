@@ -1,0 +1,141 @@
+"""Autotuner of the engine's launch parameters.
+
+The batched path has runtime knobs that trade occupancy, LDS residency and
+per-item work: the workgroup size and workgroups per CU of the HBM-resident
+launch class (`hbm_threads`, `hbm_per_cu`), shots per work item
+(`shots_per_item`), the LDS/HBM cut-over (`lds_bytes`) and, for
+large circuits, the gang shape (`gang_min_n`, `gang_chunk`, `gang_threads`).
+`tune(jobs)` times device replays of the same submission under candidate
+settings (coordinate descent from the defaults, every candidate checked to
+produce the same measurement records as the defaults), returns the fastest,
+and caches the choice per workload signature so a later run can apply it with
+`apply(engine, jobs)`.
+
+    from sdim_amd import Engine, autotune
+    best = autotune.tune(jobs)                 # dict of options + timings
+    eng = Engine(**best["options"])
+"""
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from .engine import Engine
+
+# candidate values per option, tried one option at a time from the current best
+SPACE = [
+    ("hbm_threads", (512, 256, 1024)),
+    ("hbm_per_cu", (4, 2, 1, 8)),
+    ("shots_per_item", (0, 1, 2, 4, 8, 16, 32)),
+    ("lds_bytes", (163584, 131072, 98304, 65536)),
+    ("gang_min_n", (1536, 1024, 2048, 4096, 0)),
+    ("gang_chunk", (128, 256, 64)),
+    ("gang_threads", (512, 256, 1024)),
+]
+DEFAULTS = {"hbm_threads": 512, "hbm_per_cu": 4, "shots_per_item": 0, "lds_bytes": 163584,
+            "gang_min_n": 1536, "gang_chunk": 128, "gang_threads": 512}
+# options that only matter for some workloads
+_GANG = ("gang_min_n", "gang_chunk", "gang_threads")
+
+
+def signature(jobs):
+    """Workload signature: the multiset of (n, d, shots, op count) plus a hash of the IR."""
+    h = hashlib.sha1()
+    shape = sorted((int(j.n), int(j.d), int(j.shots), int(np.asarray(j.ir).size // 3)) for j in jobs)
+    h.update(json.dumps(shape).encode())
+    for j in jobs[:64]:
+        h.update(np.ascontiguousarray(np.asarray(j.ir, dtype=np.int64)).tobytes())
+    return h.hexdigest()[:16]
+
+
+def _cache_path(path=None):
+    if path:
+        return path
+    base = os.environ.get("SDIM_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "sdim_amd")
+    return os.path.join(base, "tuning.json")
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _records(tk, n_jobs):
+    return [tk.records(i).tobytes() for i in range(n_jobs)]
+
+
+def _time(options, jobs, reps, ref=None):
+    eng = Engine(**options)
+    try:
+        tk = eng.submit(jobs).wait()
+        if ref is not None and _records(tk, len(jobs)) != ref:
+            raise RuntimeError(f"options {options} changed the measurement records")
+        recs = _records(tk, len(jobs)) if ref is None else None
+        tk.replay(1)
+        ms = tk.replay(reps) / reps
+        tk.release()
+        return ms, recs
+    finally:
+        eng.close()
+
+
+def tune(jobs, reps=3, space=SPACE, cache=None, verbose=False, max_seconds=120.0):
+    """Coordinate descent over `space` from the defaults; returns
+    {"options": best options, "ms": best device ms per replay, "default_ms": ..., "trials": [...]}.
+    Every trial must reproduce the default's records bit for bit."""
+    t_start = time.perf_counter()
+    has_gang = any(int(j.n) >= 1024 for j in jobs)
+    best = dict(DEFAULTS)
+    best_ms, ref = _time(best, jobs, reps)
+    default_ms = best_ms
+    trials = [{"options": dict(best), "ms": best_ms}]
+    for key, values in space:
+        if key in _GANG and not has_gang:
+            continue
+        for v in values:
+            if v == best[key] or time.perf_counter() - t_start > max_seconds:
+                continue
+            cand = dict(best)
+            cand[key] = v
+            try:
+                ms, _ = _time(cand, jobs, reps, ref)
+            except (ValueError, RuntimeError) as e:       # infeasible (e.g. LDS too small) or wrong
+                trials.append({"options": cand, "error": str(e)})
+                continue
+            trials.append({"options": cand, "ms": ms})
+            if verbose:
+                print(f"[tune] {key}={v}: {ms:.3f} ms (best {best_ms:.3f})", flush=True)
+            if ms < best_ms * 0.98:
+                best, best_ms = cand, ms
+    out = {"options": best, "ms": best_ms, "default_ms": default_ms, "trials": trials,
+           "signature": signature(jobs)}
+    path = _cache_path(cache)
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        db = _load(path)
+        db[out["signature"]] = {"options": best, "ms": best_ms, "default_ms": default_ms}
+        with open(path, "w") as f:
+            json.dump(db, f, indent=1)
+    except OSError:
+        pass
+    return out
+
+
+def cached(jobs, cache=None):
+    """The cached options of this workload, or None."""
+    entry = _load(_cache_path(cache)).get(signature(jobs))
+    return entry["options"] if entry else None
+
+
+def apply(engine, jobs, cache=None):
+    """Set the cached options of this workload on `engine`; returns them (or None)."""
+    opts = cached(jobs, cache)
+    if opts:
+        for k, v in opts.items():
+            engine.set_option(k, v)
+    return opts

@@ -160,7 +160,7 @@ struct sdim_ctx {
   int64_t grid = 0;
   int64_t allow_lds = 1;
   int64_t hbm_threads = 512;
-  int64_t hbm_per_cu = 2;
+  int64_t hbm_per_cu = 4;     // LDS budget of the HBM class = 160 KB / 4 (chosen by the autotuner: leaves room for LDS-class workgroups)
   int64_t gang_min_n = 1536;  // circuits with at least this many qudits run on gangs (0: never)
   int64_t gang_chunk = 128;   // minimum bytes of each member's share of a row (one cache line)
   int64_t gang_threads = 512;

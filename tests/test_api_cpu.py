@@ -195,3 +195,28 @@ def test_c_abi_exports_every_declared_symbol():
     # loads through the package binding as well
     L = _lib.load()
     assert L.sdim_last_error() is not None
+
+
+def test_autotune_signature_and_cache(tmp_path):
+    from sdim_amd import autotune
+    from sdim_amd.engine import Job
+    rng = np.random.default_rng(0)
+    jobs = [Job(5, 3, rng.integers(5, 8, size=(10, 3)).astype(np.int64) * [1, 0, 0] + [0, 1, -1], shots=4)]
+    other = [Job(6, 3, jobs[0].ir, shots=4)]
+    assert autotune.signature(jobs) == autotune.signature(list(jobs))
+    assert autotune.signature(jobs) != autotune.signature(other)
+    path = str(tmp_path / "t.json")
+    assert autotune.cached(jobs, path) is None
+    with open(path, "w") as f:
+        f.write('{"%s": {"options": {"hbm_threads": 256}, "ms": 1.0, "default_ms": 2.0}}' % autotune.signature(jobs))
+    assert autotune.cached(jobs, path) == {"hbm_threads": 256}
+
+    class FakeEngine:
+        def __init__(self):
+            self.set = {}
+
+        def set_option(self, k, v):
+            self.set[k] = v
+    e = FakeEngine()
+    assert autotune.apply(e, jobs, path) == {"hbm_threads": 256} and e.set == {"hbm_threads": 256}
+    assert set(autotune.DEFAULTS) == {k for k, _ in autotune.SPACE}

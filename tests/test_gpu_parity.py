@@ -131,3 +131,22 @@ def test_repetition_configuration(mode):
     assert rec.shape == (340, 4096)
     tk.release()
     eng.close()
+
+
+def test_autotune_keeps_records(tmp_path):
+    """The autotuner times candidate launch settings on a small mixed batch; every candidate
+    must reproduce the default's records, and the chosen options run bit-exact."""
+    from sdim_amd import autotune
+    from sdim_amd.engine import Engine
+    rng = np.random.default_rng(31)
+    jobs = [clifford_job(rng, n, d, 3 * n, shots=4, stream=i) for i, (n, d) in
+            enumerate([(40, 2), (90, 3), (230, 5), (300, 7), (20, 11)])]
+    space = [("hbm_threads", (512, 256)), ("shots_per_item", (0, 1, 2)), ("lds_bytes", (163584, 65536))]
+    out = autotune.tune(jobs, reps=1, space=space, cache=str(tmp_path / "t.json"))
+    assert out["ms"] > 0 and all("error" not in t for t in out["trials"])
+    assert autotune.cached(jobs, str(tmp_path / "t.json")) == out["options"]
+    eng = Engine(**out["options"])
+    tk = eng.run(jobs)
+    check_ticket(tk, jobs, tableaux=False)
+    tk.release()
+    eng.close()
