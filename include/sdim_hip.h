@@ -90,6 +90,8 @@ const char* sdim_last_error(void);
  *                     one cache line, so that no two members write the same line)
  *   "gang_threads"    workgroup size of the gang kernel: 256, 512 (default) or 1024
  *   "gang_max"        most workgroups per gang (0 = one per CU)
+ *   "frame_lds_bytes" frame sampler: LDS for the frames of one workgroup's 256 shots (one
+ *                     byte per qudit and shot; default 65536); larger frames live in device memory
  */
 int sdim_set_option(sdim_ctx* ctx, const char* key, int64_t value);
 int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
@@ -100,6 +102,23 @@ int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
  * ticket; results are available after sdim_wait.
  */
 int sdim_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket);
+/*
+ * Pauli-frame sampling of `n_jobs` circuits: the reference's default path for shots > 1
+ * (Program.simulate, sdim/program.py:244-265, and simulate_frame, :45-165).  refs[j]
+ * holds the reference shot's records of job j (n_meas 33-byte MEASUREMENT_DTYPE records
+ * in event order, as sdim_fetch_records writes them for a one-shot, N1-free tableau run;
+ * the reference's `reference_results`).  Each of the job's `shots` further shots tracks
+ * a Pauli frame: x = 0 and z uniform on every qudit, conjugated by every gate; M / M_X
+ * report (reference value + x_q) mod d with the reference's deterministic flag and draw
+ * z_q again, RESET reports the reference record and sets x_q = 0 with a fresh z_q, N1
+ * adds its (a, b) exponents.  Shots are numbered shot0, shot0 + 1, ... (records and
+ * Philox counters); draws: initial z_q counter (q, shot, 1, 0), redraw after the
+ * measurement at instruction k (k, shot, 1, 1), N1 (k, shot, 0, 0) as in the tableau
+ * path.  Jobs start from |0...0> (init must be NULL).  Records are fetched with
+ * sdim_num_measurements / sdim_fetch_records; such a ticket keeps no tableaux.
+ */
+int sdim_frame_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
+                      uint64_t* ticket);
 int sdim_wait(sdim_ctx* ctx, uint64_t ticket);
 int sdim_poll(sdim_ctx* ctx, uint64_t ticket, int32_t* done);
 int sdim_release(sdim_ctx* ctx, uint64_t ticket);

@@ -65,3 +65,39 @@ def simulate(n, d, ops, shots=1, seed=0, stream=0, noise=None, keep_shots=0, sho
             dph = b[4 * nn + n:4 * nn + 2 * n]
             tabs.append((x, z, ph, dx, dz, dph))
     return rec[:, :n_meas, :], tabs
+
+
+def frame(n, d, ops, ref, shots, seed=0, stream=0, noise=None, shot0=1):
+    """Same contract as oracle.frame.simulate_frame, computed in C.
+
+    ref: (value, deterministic) per measurement event of the reference shot.
+    Returns int64 records [shots, n_meas, 5] = (q, round, shot, det, value).
+    """
+    L = lib()
+    if not hasattr(L, "_frame_ready"):
+        L.oracle_frame.restype = ctypes.c_int
+        L.oracle_frame.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64,
+            ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L._frame_ready = True
+    ops = np.ascontiguousarray(np.asarray(ops, dtype=np.int64).reshape(-1, 3))
+    n_ops = ops.shape[0]
+    n_meas = int(np.isin(ops[:, 0], (14, 15, 16)).sum())
+    if len(ref) != n_meas:
+        raise ValueError("one reference record per measurement event")
+    refb = np.zeros(max(n_meas, 1), dtype=np.uint8)
+    for e, (v, det) in enumerate(ref):
+        refb[e] = (int(v) & 0x7F) | (0x80 if det else 0)
+    chan = np.zeros(max(n_ops, 1), dtype=np.int32)
+    keep = np.zeros(max(n_ops, 1), dtype=np.uint64)
+    for k, (c, k53) in (noise or {}).items():
+        chan[k] = c
+        keep[k] = k53
+    rec = np.zeros((shots, max(n_meas, 1), 5), dtype=np.int64)
+    rc = L.oracle_frame(n, d, n_ops, ops.ctypes.data, chan.ctypes.data, keep.ctypes.data,
+                        shot0, shots, seed, stream, n_meas, refb.ctypes.data, rec.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle_frame failed: {rc}")
+    return rec[:, :n_meas, :]

@@ -15,6 +15,7 @@
  *   random measurement       sdim/tableau/tableau_prime.py:294-334
  *   deterministic measurement sdim/tableau/tableau_prime.py:336-363
  *   shot loop, RESET fix-up  sdim/program.py:269-365
+ *   Pauli-frame sampler      sdim/program.py:45-165 (oracle_frame, at the end)
  *
  * Differences from the literal Python oracle (oracle/tableau.py), all of which
  * leave every result identical mod d (mod 2d for the d=2 phase):
@@ -314,5 +315,93 @@ int oracle_simulate(int n, int d, int64_t n_ops, const int64_t* ops,
             memcpy(tab_out + (size_t)s * tab_words, buf, sizeof(int32_t) * tab_words);
     }
     free(buf); free(scratch); free(counts);
+    return rc;
+}
+
+/*
+ * Pauli-frame sampler (sdim/program.py:45-165): the restatement of oracle/frame.py
+ * in C, for sizes the Python loops do not finish in seconds.
+ *   ref      [n_meas] reference records: value | det << 7 (one byte per event)
+ *   shots    frame shots, numbered shot0 .. shot0 + shots - 1
+ *   rec_out  [shots][n_meas][5] int64 (q, round, shot, det, value)
+ * Draws: initial z_q counter (q, shot, 1, 0); redraw after the measurement at
+ * instruction k (k, shot, 1, 1); N1 (k, shot, 0, 0) as in oracle_simulate.
+ */
+static uint32_t frame_draw(uint64_t seed, uint64_t stream, uint64_t c0, uint64_t shot, uint64_t kind, int d) {
+    uint64_t ctr[4] = {c0, shot, 1, kind};
+    uint64_t key[2] = {seed, stream};
+    uint64_t out[4];
+    philox4x64_10(ctr, key, out);
+    return (uint32_t)mulhi64(out[0], (uint64_t)d);
+}
+
+int oracle_frame(int n, int d, int64_t n_ops, const int64_t* ops,
+                 const int32_t* noise_channel, const uint64_t* noise_keep53,
+                 int64_t shot0, int64_t shots, uint64_t seed, uint64_t stream,
+                 int64_t n_meas, const uint8_t* ref, int64_t* rec_out) {
+    if (n < 1 || d < 2) return -1;
+    int32_t* x = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* z = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!x || !z || !counts) { free(x); free(z); free(counts); return -2; }
+    int rc = 0;
+    for (int64_t s = 0; s < shots && rc == 0; s++) {
+        const uint64_t shot = (uint64_t)(shot0 + s);
+        for (int q = 0; q < n; q++) { x[q] = 0; z[q] = (int32_t)frame_draw(seed, stream, (uint64_t)q, shot, 0, d); counts[q] = 0; }
+        int64_t e = 0;
+        for (int64_t k = 0; k < n_ops && rc == 0; k++) {
+            const int64_t gid = ops[3 * k];
+            const int q = (int)ops[3 * k + 1], t = (int)ops[3 * k + 2];
+            if (q < 0 || q >= n || ((gid >= 9 && gid <= 13) && (t < 0 || t >= n))) { rc = -3; break; }
+            int32_t tmp;
+            switch (gid) {
+            case 5: tmp = x[q]; x[q] = pmod(-(int64_t)z[q], d); z[q] = tmp; break;
+            case 6: tmp = x[q]; x[q] = z[q]; z[q] = pmod(-(int64_t)tmp, d); break;
+            case 7: z[q] = pmod((int64_t)z[q] + x[q], d); break;
+            case 8: z[q] = pmod((int64_t)z[q] - x[q], d); break;
+            case 9: x[t] = pmod((int64_t)x[t] + x[q], d); z[q] = pmod((int64_t)z[q] - z[t], d); break;
+            case 10: x[t] = pmod((int64_t)x[t] - x[q], d); z[q] = pmod((int64_t)z[q] + z[t], d); break;
+            case 11: z[t] = pmod((int64_t)z[t] + x[q], d); z[q] = pmod((int64_t)z[q] + x[t], d); break;
+            case 12: z[t] = pmod((int64_t)z[t] - x[q], d); z[q] = pmod((int64_t)z[q] - x[t], d); break;
+            case 13: tmp = x[q]; x[q] = x[t]; x[t] = tmp; tmp = z[q]; z[q] = z[t]; z[t] = tmp; break;
+            case 14: case 15: case 16: {
+                if (e >= n_meas) { rc = -4; break; }
+                const int64_t rv = ref[e] & 0x7F, det = (ref[e] >> 7) & 1;
+                int64_t* r = rec_out + ((size_t)s * n_meas + e) * 5;
+                r[0] = q; r[1] = counts[q]; r[2] = (int64_t)shot; r[3] = det;
+                if (gid == 16) {
+                    r[4] = rv;
+                    x[q] = 0;
+                } else {
+                    if (gid == 15) { tmp = x[q]; x[q] = z[q]; z[q] = pmod(-(int64_t)tmp, d); }
+                    r[4] = pmod(rv + x[q], d);
+                }
+                z[q] = (int32_t)frame_draw(seed, stream, (uint64_t)k, shot, 1, d);
+                counts[q]++; e++;
+                break;
+            }
+            case 17: {
+                if (!noise_channel || !noise_keep53) { rc = -5; break; }
+                uint64_t blk[4];
+                draw_block(seed, stream, (uint64_t)k, shot, blk);
+                if ((blk[0] >> 11) < noise_keep53[k]) break;
+                int64_t a = 0, b = 0;
+                const int ch = noise_channel[k];
+                if (ch == 0) { int64_t r = 1 + (int64_t)mulhi64(blk[1], (uint64_t)(d * d - 1)); a = r % d; b = r / d; }
+                else if (ch == 1) { a = 1 + (int64_t)mulhi64(blk[1], (uint64_t)(d - 1)); }
+                else if (ch == 2) { b = 1 + (int64_t)mulhi64(blk[1], (uint64_t)(d - 1)); }
+                else { rc = -6; break; }
+                x[q] = pmod(x[q] + a, d);
+                z[q] = pmod(z[q] + b, d);
+                break;
+            }
+            default:
+                if (gid < 0 || gid > 17) rc = -7;
+                break;   /* I, X, Z and inverses leave the frame unchanged */
+            }
+        }
+        if (rc == 0 && e != n_meas) rc = -8;
+    }
+    free(x); free(z); free(counts);
     return rc;
 }

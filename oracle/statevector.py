@@ -154,3 +154,36 @@ def stabilizes(psi, xcol, zcol, phase, d, atol=1e-9):
 def symplectic(x1, z1, x2, z2, d):
     """Commutation exponent k with A B = w^k B A for A = X^x1 Z^z1, B = X^x2 Z^z2."""
     return int((np.dot(z1, x2) - np.dot(x1, z2)) % d)
+
+
+def outcome_distribution(n, d, ops, tol=1e-9):
+    """Exact joint distribution of the measurement records of a noiseless circuit:
+    {tuple of values per measurement event: probability}, by branching on every
+    outcome of every M, M_X (H^-1 then a Z measurement, tableau_gates.py:280-296)
+    and RESET (measure, then X^(-m), program.py:334-339)."""
+    psi0 = np.zeros(d ** n, dtype=complex)
+    psi0[0] = 1.0
+    out = {}
+
+    def walk(psi, k, prob, rec):
+        while k < len(ops):
+            gid, q, t = (int(v) for v in ops[k])
+            if gid in (14, 15, 16):
+                if gid == 15:
+                    psi = apply_gate(psi, 6, q, -1, n, d)
+                p = outcome_probs(psi, q, n, d)
+                for m in range(d):
+                    if p[m] > tol:
+                        nxt = project(psi, q, m, n, d)
+                        if gid == 16:
+                            for _ in range((-m) % d):
+                                nxt = apply_gate(nxt, 1, q, -1, n, d)
+                        walk(nxt, k + 1, prob * p[m], rec + (m,))
+                return
+            if gid not in (0, 17):
+                psi = apply_gate(psi, gid, q, t, n, d)
+            k += 1
+        out[rec] = out.get(rec, 0.0) + prob
+
+    walk(psi0, 0, 1.0, ())
+    return out

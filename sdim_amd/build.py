@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsdim_hip.so")
-SOURCES = ["engine.hip", "runtime.cpp"]
+SOURCES = ["engine.hip", "frame.hip", "runtime.cpp"]
 HEADERS = ["engine.h", "sdim_types.h", "sdim_arith.h"]
 
 

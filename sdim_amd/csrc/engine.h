@@ -77,6 +77,17 @@ __host__ __device__ inline uint64_t gang_scratch_bytes(uint32_t K, uint32_t n_ma
          2 * gang_stage_bytes(n_max, S_max);
 }
 
+// ---- Pauli-frame sampler (frame.hip): one workgroup per item of up to FRAME_NT shots
+#define FRAME_NT 256
+struct FrameParams {
+  const ItemDesc* items;
+  const DevOp* ops;
+  uint8_t* arena;      // records at it.rec_off, device-memory frames at it.tab_off
+  uint32_t nt;
+};
+int launch_frame(const FrameParams& P, uint32_t n_items, uint32_t lds_bytes, hipStream_t st);
+int prepare_frame_kernel();
+
 int prepare_kernels();
 int launch_gang(const KParams& P, const GangParams& G, int d, int nt, int grid, hipStream_t st);
 int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);

@@ -63,6 +63,7 @@ struct Prim {
 struct JobPlan {
   int32_t n = 0, d = 0;
   int64_t shots = 0;
+  int64_t shot_base = 0;     // shot index of the job's first shot (frame sampling starts after the reference shot)
   int64_t n_meas = 0;
   std::vector<int32_t> meas_q, meas_round;
   uint32_t S_item = 1;
@@ -140,7 +141,13 @@ struct Ticket {
   std::vector<LaunchClass> classes;
   int n_cu = 256, grid_override = 0, threads = 0;
   hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+  // frame-sampler tickets
+  FrameParams FP{};
+  uint32_t frame_lds = 0;
 };
+
+// ticket mode of frame-sampler submissions (sdim_frame_submit)
+constexpr int MODE_FRAME = 2;
 
 }  // namespace
 
@@ -165,6 +172,7 @@ struct sdim_ctx {
   int64_t gang_chunk = 128;   // minimum bytes of each member's share of a row (one cache line)
   int64_t gang_threads = 512;
   int64_t gang_max = 0;       // most members per gang (0: every CU)
+  int64_t frame_lds_bytes = 65536;  // frame sampler: LDS for the frames of one workgroup (else device memory)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
   std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
 };
@@ -276,6 +284,52 @@ DevOp mkop(uint32_t kind, uint32_t a, uint32_t b, uint32_t q, uint32_t t, uint32
   o.t = t;
   o.src = src;
   return o;
+}
+
+// Frame-sampler lowering (sdim/program.py:75-165): gates that change a frame, the
+// reference record of every measurement event (checked against the event's qudit
+// and round), and each N1's keep53 threshold inside its op.
+int frame_expand(const sdim_job& j, const uint8_t* ref, JobPlan& jp, std::vector<DevOp>& ops) {
+  const int64_t n = j.n;
+  const uint32_t d = (uint32_t)j.d;
+  std::vector<int32_t> counts((size_t)n, 0);
+  jp.meas_q.clear();
+  jp.meas_round.clear();
+  for (int64_t k = 0; k < j.n_ops; k++) {
+    const int64_t gid = j.ir[3 * k], q = j.ir[3 * k + 1], t = j.ir[3 * k + 2];
+    if (gid < 0 || gid > 17) return fail(SDIM_EINVAL, "Invalid gate value");
+    if (q < 0 || q >= n) return fail(SDIM_EINVAL, "qudit index out of range at op " + std::to_string(k));
+    const bool two = (gid >= 9 && gid <= 13);
+    if (two && (t < 0 || t >= n)) return fail(SDIM_EINVAL, "target index out of range at op " + std::to_string(k));
+    DevOp o{(uint32_t)gid, (uint32_t)q, two ? (uint32_t)t : 0u, (uint32_t)k};
+    if (gid <= 4) continue;                       // I, X, Z and inverses leave a frame unchanged
+    if (gid >= 14 && gid <= 16) {
+      const int64_t e = (int64_t)jp.meas_q.size();
+      const uint8_t* r = ref + 33 * (size_t)e;
+      int64_t rq, rround, rval;
+      memcpy(&rq, r, 8);
+      memcpy(&rround, r + 8, 8);
+      memcpy(&rval, r + 25, 8);
+      if (rq != q || rround != counts[(size_t)q] || rval < 0 || rval >= (int64_t)d || r[24] > 1)
+        return fail(SDIM_EINVAL, "reference record " + std::to_string(e) + " does not match measurement op " +
+                                     std::to_string(k) + " (qudit " + std::to_string(q) + ", round " +
+                                     std::to_string(counts[(size_t)q]) + ")");
+      o.w0 |= (uint32_t)(rval | (r[24] << 7)) << 8;
+      o.t = (uint32_t)e;
+      jp.meas_q.push_back((int32_t)q);
+      jp.meas_round.push_back(counts[(size_t)q]++);
+    } else if (gid == 17) {
+      if (!j.noise_channel || !j.noise_prob) return fail(SDIM_EINVAL, "N1 op without noise parameters");
+      const int32_t ch = j.noise_channel[k];
+      if (ch < 0 || ch > 2) return fail(SDIM_EINVAL, "Must specify a valid noise channel.");
+      const uint64_t keep = keep_threshold53(j.noise_prob[k]);
+      o.w0 |= ((uint32_t)ch << 8) | ((uint32_t)(keep >> 32) << 10);
+      o.t = (uint32_t)(keep & 0xFFFFFFFFull);
+    }
+    ops.push_back(o);
+  }
+  jp.n_meas = (int64_t)jp.meas_q.size();
+  return 0;
 }
 
 // segments + ASAP layers for the batch kernel
@@ -457,6 +511,12 @@ void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::itera
 // otherwise hold the CUs the other one needs.
 int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
   KParams& P = T->P;
+  if (T->mode == MODE_FRAME) {
+    if (which != 0) return fail(SDIM_EINVAL, "a frame-sampler ticket has no item classes");
+    if (launch_frame(T->FP, (uint32_t)T->items.size(), T->frame_lds, T->stream))
+      return fail(SDIM_EHIP, "frame kernel launch failed");
+    return 0;
+  }
   if (T->mode == SDIM_MODE_BATCHED) {
     if (launch_reset(T->d_queue, P.counters, T->stream)) return fail(SDIM_EHIP, "reset launch failed");
     // batch launches run as parallel branches (fork/join through events: parallel nodes
@@ -563,7 +623,7 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
     }
   }
   HIP_OK(hipEventCreateWithFlags(&c->gang_done, hipEventDisableTiming));
-  if (prepare_kernels()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
+  if (prepare_kernels() || prepare_frame_kernel()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
   *out = c.release();
   return 0;
 }
@@ -627,6 +687,9 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "gang_max") {
     if (v < 0) return fail(SDIM_EINVAL, "gang_max must be >= 0");
     c->gang_max = v;
+  } else if (k == "frame_lds_bytes") {
+    if (v < 0 || v > 163840) return fail(SDIM_EINVAL, "frame_lds_bytes must be in [0, 163840]");
+    c->frame_lds_bytes = v;
   } else {
     return fail(SDIM_EINVAL, "unknown option " + k);
   }
@@ -649,6 +712,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "gang_chunk") *v = c->gang_chunk;
   else if (k == "gang_threads") *v = c->gang_threads;
   else if (k == "gang_max") *v = c->gang_max;
+  else if (k == "frame_lds_bytes") *v = c->frame_lds_bytes;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -989,6 +1053,110 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   return submit_locked(c, jobs, n_jobs, mode, 0, ticket, nullptr);
 }
 
+int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
+                      uint64_t* ticket) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  if ((!jobs && n_jobs) || n_jobs < 0 || !ticket || (n_jobs && !refs)) return fail(SDIM_EINVAL, "bad submit arguments");
+  if (shot0 < 0) return fail(SDIM_EINVAL, "shot0 must be >= 0");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_OK(hipSetDevice(c->device));
+  std::unique_ptr<Ticket> T(new Ticket());
+  T->mode = MODE_FRAME;
+  T->stream = c->stream;
+  T->jobs.resize((size_t)n_jobs);
+  std::vector<DevOp> ops;
+  std::vector<ItemDesc>& items = T->items;
+  const uint32_t NT = FRAME_NT;
+  uint64_t rec_total = 0;
+  uint32_t lds = 0;
+  for (int32_t ji = 0; ji < n_jobs; ji++) {
+    const sdim_job& j = jobs[ji];
+    JobPlan& jp = T->jobs[(size_t)ji];
+    if (j.n < 1) return fail(SDIM_EINVAL, "Number of qudits must be greater than 0");
+    if (j.d < 2) return fail(SDIM_EINVAL, "Dimension must be greater than 1");
+    if (!supported_dim(j.d)) return fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})");
+    if (j.shots < 1) return fail(SDIM_EINVAL, "shots must be >= 1");
+    if (shot0 + j.shots > 0xFFFFFFFFll) return fail(SDIM_EINVAL, "shot indices must stay below 2^32");
+    if (j.n > 32768) return fail(SDIM_EINVAL, "at most 32768 qudits are supported");
+    if (j.n_ops < 0 || (j.n_ops && !j.ir)) return fail(SDIM_EINVAL, "bad op list");
+    if (j.init[0] || j.init[1] || j.init[2] || j.init[3] || j.init[4] || j.init[5])
+      return fail(SDIM_EINVAL, "the frame sampler starts from |0...0> (its frames start as random Z)");
+    jp.n = j.n;
+    jp.d = j.d;
+    jp.shots = j.shots;
+    jp.shot_base = shot0;
+    jp.ops_off = (uint32_t)ops.size();
+    const uint8_t* ref = (const uint8_t*)refs[ji];
+    // events are counted before the records are read
+    int64_t n_ev = 0;
+    for (int64_t k = 0; k < j.n_ops; k++) n_ev += (j.ir[3 * k] >= 14 && j.ir[3 * k] <= 16);
+    if (n_ev && !ref) return fail(SDIM_EINVAL, "missing reference records");
+    int rc = frame_expand(j, ref, jp, ops);
+    if (rc) return rc;
+    jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
+    const uint32_t n = (uint32_t)j.n;
+    const bool in_lds = (uint64_t)n * NT <= (uint64_t)c->frame_lds_bytes;
+    if (in_lds) lds = std::max<uint32_t>(lds, (n * NT + 15u) & ~15u);
+    jp.first_item = (uint32_t)items.size();
+    for (int64_t s0 = 0; s0 < j.shots; s0 += NT) {
+      ItemDesc it;
+      memset(&it, 0, sizeof(it));
+      it.seed = j.seed;
+      it.stream = j.stream;
+      it.ops_off = jp.ops_off;
+      it.n_ops = jp.n_ops;
+      it.n = n;
+      it.d = (uint32_t)j.d;
+      it.S = (uint32_t)std::min<int64_t>(NT, j.shots - s0);
+      it.shot0 = (uint32_t)(shot0 + s0);
+      it.flags = in_lds ? ITEM_LDS : 0;
+      it.n_meas = (uint32_t)jp.n_meas;
+      it.rec_off = rec_total;
+      rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
+      items.push_back(it);
+    }
+    jp.n_items = (uint32_t)items.size() - jp.first_item;
+  }
+  // arena: records, then the device-memory frames of items that do not use LDS
+  T->rec_bytes = rec_total;
+  uint64_t off = align256(rec_total);
+  for (auto& it : items) {
+    if (it.flags & ITEM_LDS) continue;
+    it.tab_off = off;
+    off = align256(off + (uint64_t)it.n * NT);
+  }
+  const uint64_t o_items = 0;
+  const uint64_t o_ops = align256(o_items + items.size() * sizeof(ItemDesc));
+  const uint64_t meta_bytes = std::max<uint64_t>(align256(o_ops + ops.size() * sizeof(DevOp)), 256);
+  int rc = ctx_alloc(c, std::max<uint64_t>(off, 256), T->arena);
+  if (rc) return rc;
+  rc = ctx_alloc(c, meta_bytes, T->meta);
+  if (rc) { ctx_free(c, T->arena); return rc; }
+  uint8_t* meta = (uint8_t*)T->meta.p;
+  T->h_meta.assign(meta_bytes, 0);
+  if (!items.empty()) memcpy(T->h_meta.data() + o_items, items.data(), items.size() * sizeof(ItemDesc));
+  if (!ops.empty()) memcpy(T->h_meta.data() + o_ops, ops.data(), ops.size() * sizeof(DevOp));
+  HIP_OK(hipMemcpyAsync(meta, T->h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
+  T->FP.items = (const ItemDesc*)(meta + o_items);
+  T->FP.ops = (const DevOp*)(meta + o_ops);
+  T->FP.arena = (uint8_t*)T->arena.p;
+  T->FP.nt = NT;
+  T->frame_lds = lds;
+  for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
+  HIP_OK(hipEventCreate(&T->ev0));
+  HIP_OK(hipEventCreate(&T->ev1));
+  HIP_OK(hipEventRecord(T->ev0, T->stream));
+  rc = enqueue(T.get(), 0);
+  if (rc) return rc;
+  HIP_OK(hipEventRecord(T->ev1, T->stream));
+  T->h_rec.resize(T->rec_bytes);
+  if (T->rec_bytes) HIP_OK(hipMemcpyAsync(T->h_rec.data(), T->arena.p, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
+  T->id = c->next_ticket++;
+  *ticket = T->id;
+  c->tickets[T->id] = std::move(T);
+  return 0;
+}
+
 int sdim_wait(sdim_ctx* c, uint64_t ticket) {
   if (!c) return fail(SDIM_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -1047,7 +1215,7 @@ int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int
       const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
       for (uint32_t s = 0; s < it.S; s++) {
         const int64_t shot = (int64_t)it.shot0 + s;
-        uint8_t* rec = o + 33 * (size_t)(ev * jp.shots + shot);
+        uint8_t* rec = o + 33 * (size_t)(ev * jp.shots + (shot - jp.shot_base));
         const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], val = r[s] & 0x7F;
         const uint8_t det = (r[s] >> 7) & 1;
         memcpy(rec, &q, 8);
@@ -1071,6 +1239,7 @@ int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, 
   int rc = finish(c, t);
   if (rc) return rc;
   const JobPlan& jp = t->jobs[(size_t)job];
+  if (t->mode == MODE_FRAME) return fail(SDIM_EINVAL, "a frame-sampled job keeps no tableaux");
   if (shot < 0 || shot >= jp.shots) return fail(SDIM_EINVAL, "shot out of range");
   for (uint32_t i = 0; i < jp.n_items; i++) {
     const ItemDesc& it = t->items[jp.first_item + i];
@@ -1168,7 +1337,8 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
     }
   }
   HIP_OK(hipEventRecord(t->ev1, t->stream));
-  HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, sizeof(t->h_counters), hipMemcpyDeviceToHost, t->stream));
+  if (t->d_counters)
+    HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, sizeof(t->h_counters), hipMemcpyDeviceToHost, t->stream));
   if (t->rec_bytes)
     HIP_OK(hipMemcpyAsync(t->h_rec.data(), t->arena.p, t->rec_bytes, hipMemcpyDeviceToHost, t->stream));
   HIP_OK(hipStreamSynchronize(t->stream));

@@ -126,6 +126,10 @@ SDIM_DEV U64x4 draw(uint64_t seed, uint64_t stream, uint64_t op, uint64_t shot) 
   return philox4x64_10(op, shot, 0, 0, seed, stream);
 }
 
+// Counter word 2 of the frame sampler's draws (initial and redrawn z components);
+// the tableau path's draws use 0.
+constexpr uint64_t FRAME_DOMAIN = 1;
+
 SDIM_DEV uint32_t outcome_of(const U64x4& o, uint32_t d) { return (uint32_t)__umul64hi(o.v[0], (uint64_t)d); }
 
 // (a, b) exponents of an N1 error, 0 when the coin says identity (program.py:484-503 semantics)

@@ -50,10 +50,22 @@ static inline __host__ __device__ uint32_t op_b(const DevOp& o) { return (o.w0 >
 static inline __host__ __device__ uint32_t op_c(const DevOp& o) { return (o.w0 >> 24) & 0xFFu; }
 static inline __host__ __device__ uint32_t op_noise_slot(const DevOp& o) { return (o.w0 >> 16) & 0xFFFFu; }
 
+// Ops of the Pauli-frame sampler (frame.hip): a DevOp whose kind is the gate id
+// (sdim/gatedata.py:42-102) of a gate that changes the frame; X, Z and their
+// inverses are dropped (they leave a frame unchanged).  Gates: q, t.  M, M_X,
+// RESET: byte a = the reference shot's record (value | det << 7), t = event
+// index.  N1: bits 8-9 = channel, bits 10-31 = keep53 >> 32, t = keep53 low
+// word.  src = instruction index (Philox counter word 0).
+enum FrameOpKind : uint32_t {
+  FOP_H = 5, FOP_HI = 6, FOP_P = 7, FOP_PI = 8, FOP_CNOT = 9, FOP_CNOTI = 10, FOP_CZ = 11,
+  FOP_CZI = 12, FOP_SWAP = 13, FOP_M = 14, FOP_MX = 15, FOP_RESET = 16, FOP_NOISE = 17,
+};
+
 enum ItemFlags : uint32_t {
   ITEM_LDS = 1u,        // tableau resident in LDS for the whole item
   ITEM_INIT_GIVEN = 2u, // start from the tableau already stored at tab_off (else identity)
   ITEM_PH_LDS = 4u,     // X/Z in HBM, the shots' phase vectors in LDS for the whole item
+  // frame items: ITEM_LDS = the frames of the item's shots live in LDS (else at tab_off)
 };
 
 struct ItemDesc {

@@ -181,6 +181,28 @@ class Engine:
     def run(self, jobs, mode="batched"):
         return self.submit(jobs, mode).wait()
 
+    def frame_submit(self, jobs, refs, shot0=1):
+        """Pauli-frame sampling (sdim/program.py:45-165): ``jobs[i].shots`` shots numbered from
+        ``shot0``, each reporting the reference shot's records ``refs[i]`` (MEASUREMENT_DTYPE,
+        one per measurement event in op order) shifted by its frame."""
+        jobs = list(jobs)
+        if len(refs) != len(jobs):
+            raise ValueError("one reference record array per job")
+        arr = (_lib.SdimJob * max(len(jobs), 1))()
+        ptrs = (ctypes.c_void_p * max(len(jobs), 1))()
+        keep = []
+        for i, (j, r) in enumerate(zip(jobs, refs)):
+            arr[i] = j.to_c()
+            r = np.ascontiguousarray(np.asarray(r, dtype=MEASUREMENT_DTYPE).reshape(-1))
+            keep.append(r)
+            ptrs[i] = r.ctypes.data if r.size else None
+        tid = ctypes.c_uint64(0)
+        _lib.check(self.L.sdim_frame_submit(self.ctx, arr, len(jobs), ptrs, int(shot0), ctypes.byref(tid)), self.L)
+        return Ticket(self, tid.value, jobs)
+
+    def frame_run(self, jobs, refs, shot0=1):
+        return self.frame_submit(jobs, refs, shot0).wait()
+
     def close(self):
         if self.ctx:
             self.L.sdim_ctx_destroy(self.ctx)

@@ -7,12 +7,16 @@ shapes (a flat list ordered by qudit then round for one shot, otherwise
 ``_results_to_array`` / ``_combine_results`` / ``_build_ir`` helpers with the
 reference's formats.
 
-Semantics: every shot runs the tableau algorithm (the reference's
-``force_tableau=True`` path, program.py:269-365) on the device, with random
-outcomes and N1 noise drawn from Philox4x64-10 keyed by (seed, stream) and
-(instruction index, shot).  The reference's default for ``shots > 1`` is a
-Pauli-frame sampler (program.py:45-165) whose results are only equal in
-distribution; this package always runs the tableau engine.
+Dispatch as in the reference (program.py:244-267): with ``shots > 1`` and
+neither ``record_tableau`` nor ``force_tableau``, one noiseless reference shot
+runs the tableau algorithm (N1 acts as the identity there, program.py:31) and
+the other ``shots - 1`` shots run the device Pauli-frame sampler
+(program.py:45-165); otherwise every shot runs the tableau algorithm
+(program.py:269-365) on the device.  Random outcomes, frame randomisation and
+N1 noise are drawn from Philox4x64-10 keyed by (seed, stream) and (instruction
+index, shot), so a run is reproducible from its seed.  Unlike the reference's
+tableau path, the device tableau path applies N1 noise (as X^a Z^b) in every
+shot it simulates.
 """
 import copy
 import random
@@ -64,6 +68,8 @@ class Program:
             options = SimulationOptions(shots=shots, show_measurement=show_measurement,
                                         record_tableau=record_tableau, force_tableau=force_tableau,
                                         verbose=verbose, show_gate=show_gate, exact=exact)
+        if options.shots > 1 and not options.record_tableau and not options.force_tableau:
+            return self._simulate_frame(options)
         return self._simulate_tableau(options)
 
     def _operations(self):
@@ -72,10 +78,12 @@ class Program:
             ops.extend(c.operations)
         return ops
 
-    def _job(self, ops, shots, seed, stream=0):
+    def _job(self, ops, shots, seed, stream=0, noiseless=False):
         tab = self.initial_tableau
         n, d = tab.num_qudits, tab.dimension
         ir, chan, prob = ops_to_ir(ops)
+        if noiseless:                 # N1 -> I, keeping instruction indices (Philox counters)
+            ir[ir[:, 0] == 17, 0] = 0
         init = None
         if not (isinstance(tab, ExtendedTableau) and tab.is_initial_state()):
             if not isinstance(tab, ExtendedTableau):
@@ -142,6 +150,55 @@ class Program:
             return flat
         return self.measurement_results
 
+    def _simulate_frame(self, options: SimulationOptions):
+        """sdim/program.py:244-265: a noiseless tableau reference shot (shot 0), then
+        ``shots - 1`` Pauli-frame shots (shots 1..) on the device, combined per qudit and
+        round as by ``_combine_results``."""
+        engine = self.engine or default_engine()
+        tab = self.initial_tableau
+        if not (isinstance(tab, ExtendedTableau) and tab.is_initial_state()):
+            raise NotImplementedError("the Pauli-frame sampler starts from |0...0>; use force_tableau=True "
+                                      "with an initial tableau")
+        ops = self._operations()
+        n, d = tab.num_qudits, tab.dimension
+        shots = int(options.shots)
+        seed = options.seed if options.seed is not None else (self.seed if self.seed is not None
+                                                              else random.getrandbits(63))
+        self.last_seed = seed
+        ref_job = self._job(ops, 1, seed, self.last_stream, noiseless=True)
+        tk = engine.run([ref_job])
+        ref = tk.records(0)[:, 0]
+        final = tk.tableau(0, 0)
+        tk.release()
+        if options.show_gate or options.verbose:
+            self._trace(engine, ops, seed, options, noiseless=True)
+        fj = self._job(ops, shots - 1, seed, self.last_stream)
+        tk = engine.frame_run([fj], [ref], shot0=1)
+        rec = tk.records(0)                       # (events, shots - 1)
+        tk.release()
+
+        self.measurement_results = [[] for _ in range(n)]
+        for ev in range(ref.shape[0]):
+            q = int(ref[ev]['qudit_index'])
+            rnd = int(ref[ev]['meas_round'])
+            while len(self.measurement_results[q]) <= rnd:
+                self.measurement_results[q].append([])
+            lst = self.measurement_results[q][rnd]
+            lst.append(MeasurementResult(q, bool(ref[ev]['deterministic']), int(ref[ev]['measurement_value'])))
+            dets = rec[ev]['deterministic']
+            vals = rec[ev]['measurement_value']
+            lst.extend(MeasurementResult(q, bool(dets[s]), int(vals[s])) for s in range(shots - 1))
+
+        self.stabilizer_tableau = ExtendedTableau(n, d)
+        self.stabilizer_tableau.set_arrays(final)
+        if options.show_measurement:
+            # the reference prints the reference shot only (program.py:351-353 with shots = 1)
+            print("Measurement results for shot 1:")
+            for per_q in self.measurement_results:
+                for shots_list in per_q:
+                    print(shots_list[0])
+        return self.measurement_results
+
     def _measurement_snapshots(self, engine, ops, shots, seed, mode):
         """Tableau right after each measurement event (before RESET's X fix-up), per shot
         (sdim/program.py:323-324).  Each prefix is re-simulated with the same counter-based
@@ -164,7 +221,7 @@ class Program:
             out.append(per_shot)
         return out
 
-    def _trace(self, engine, ops, seed, options):
+    def _trace(self, engine, ops, seed, options, noiseless=False):
         """``show_gate`` / ``verbose`` output of the first shot (sdim/program.py:313-350)."""
         n, d = self.stabilizer_tableau.num_qudits, self.stabilizer_tableau.dimension
         length = len(ops)
@@ -177,7 +234,7 @@ class Program:
                 info = op.target_index if op.target_index is not None else ""
                 print("Time step" if k < length - 1 else "Final step", k, "\t", op.name, op.qudit_index, info)
             if options.verbose:
-                tk = engine.run([self._job(ops[:k + 1], 1, seed, self.last_stream)])
+                tk = engine.run([self._job(ops[:k + 1], 1, seed, self.last_stream, noiseless)])
                 t = ExtendedTableau(n, d)
                 t.set_arrays(tk.tableau(0, 0))
                 tk.release()

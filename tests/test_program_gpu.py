@@ -177,7 +177,7 @@ def test_program_matches_oracle_and_final_tableau(d):
     c.add_gate("N1", 2, prob=0.5, noise_channel="d")
     c.add_gate("RESET", 2)
     prog = S.Program(c, seed=99)
-    res = prog.simulate(shots=5)
+    res = prog.simulate(shots=5, force_tableau=True)
     rec, tabs = _oracle_records(prog, 5)
     for k in range(rec.shape[1]):
         q, rnd = int(rec[0, k, 0]), int(rec[0, k, 1])
@@ -187,6 +187,35 @@ def test_program_matches_oracle_and_final_tableau(d):
     # the program keeps the last shot's final tableau (sdim/program.py:308-351)
     got = prog.stabilizer_tableau.arrays()
     for a, b in zip(got, tabs[-1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("d", [2, 3, 5])
+def test_program_frame_path(d):
+    """shots > 1 without force_tableau (sdim/program.py:244-265): shot 0 is the noiseless
+    tableau shot, shots 1.. are frame shots; the final tableau is the reference shot's."""
+    c = S.generate_random_clifford_circuit(6, 80, d, measurement_rounds=2, seed=10 + d)
+    c.add_gate("N1", 2, prob=0.5, noise_channel="d")
+    c.add_gate("M_X", 1)
+    c.add_gate("RESET", 2)
+    prog = S.Program(c, seed=77)
+    res = prog.simulate(shots=300)
+    ir, chan, prob = c.to_ir()
+    ir0 = ir.copy()
+    ir0[ir0[:, 0] == 17, 0] = 0
+    rec0, tabs = c_oracle.simulate(6, d, ir0, shots=1, seed=77, stream=0, keep_shots=1)
+    ref = [(int(r[4]), bool(r[3])) for r in rec0[0]]
+    noise = {int(k): (int(chan[k]), philox.keep_threshold53(float(prob[k]))) for k in np.nonzero(ir[:, 0] == 17)[0]}
+    fr = c_oracle.frame(6, d, ir, ref, 299, 77, 0, noise, shot0=1)
+    for k in range(rec0.shape[1]):
+        q, rnd = int(rec0[0, k, 0]), int(rec0[0, k, 1])
+        got = res[q][rnd]
+        assert len(got) == 300
+        assert (got[0].qudit_index, got[0].deterministic, got[0].measurement_value) == (q, ref[k][1], ref[k][0])
+        for s in range(299):
+            m = got[s + 1]
+            assert (m.qudit_index, m.deterministic, m.measurement_value) == (q, bool(fr[s, k, 3]), int(fr[s, k, 4]))
+    for a, b in zip(prog.stabilizer_tableau.arrays(), tabs[0]):
         assert np.array_equal(a, b)
 
 
@@ -231,7 +260,9 @@ def test_initial_tableau_and_extended_tableau_methods():
     c = S.Circuit(3, 5)
     c.add_gate("M", [0, 1, 2])
     prog = S.Program(c, tableau=t, seed=7)
-    res = prog.simulate(shots=3)
+    with pytest.raises(NotImplementedError):
+        prog.simulate(shots=3)                # the frame sampler starts from |0...0>
+    res = prog.simulate(shots=3, force_tableau=True)
     ops = [(14, 0, -1), (14, 1, -1), (14, 2, -1)]
     rec, _ = OP.simulate(3, 5, ops, shots=3, seed=7, stream=0, initial=o)
     got = [(q, rnd, s, m.deterministic, m.measurement_value)
