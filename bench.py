@@ -15,6 +15,10 @@ Besides the headline (batched persistent path) the line carries:
   roofline_lds   LDS roofline of the kernel pass over LDS-resident tableaux
   large_tableau  the large-tableau configuration (d=5, n=8192, one circuit on a gang
                  of workgroups) with its HBM roofline
+  frame          the reference's default shots > 1 path on the repetition-code
+                 configuration: a noiseless tableau reference shot + 4095 Pauli-frame
+                 shots on the device, alone and as 64 circuits per launch, with the
+                 frame oracle (C, 1 core) beside it
   serving        Poisson arrivals with HBM-budget admission: p50/p99 latency, aggregate rate
   cpu_baseline   the CPU oracle (literal restatement of the reference's
                  Program.simulate tableau path, same interpreted cost structure),
@@ -32,6 +36,8 @@ METRIC = "gate-shots/s, batched vs plain on 1 MI355X (bit-exact); p50/p99 latenc
 HBM_PEAK_GBS = 8000.0
 # 256 CUs x 128 B/clk x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
+# full-rate wave64 VALU instructions: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -39,7 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="uneven", choices=["uneven", "small", "repetition"])
+    ap.add_argument("--workload", default="uneven", choices=["uneven", "small", "repetition", "frame"],
+                    help="frame: only the Pauli-frame leg (repetition-code configuration)")
     ap.add_argument("--plain-every", type=int, default=16, help="plain path runs every k-th circuit")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -48,6 +55,7 @@ def parse():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner on the workload first and measure with its choice")
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -59,17 +67,22 @@ def parse():
     return ap.parse_args()
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated against a
-    plain copy) in profiles/<name>.md, or None."""
+def profiled_value(name, key):
+    """A figure recorded from a rocprofv3 run in profiles/<name>.md ("key: value"), or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name + ".md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    plain copy) in profiles/<name>.md, or None."""
+    return profiled_value(name, "traffic_bytes_per_launch")
 
 
 def copy_calibration():
@@ -180,6 +193,81 @@ def large_tableau(eng, steps, warmup):
                          "algorithmic_bytes": st2["bytes_hbm"], "kernel_ms": round(ms, 3)}}
 
 
+def frame_sampler(eng, steps, warmup, cpu_seconds):
+    """Repetition-code configuration (d=3, 17+17 qutrits, 10 rounds, p=1e-3, 4096 shots) through
+    the reference's default shots > 1 path (sdim/program.py:244-265): shot 0 is a noiseless
+    tableau shot, shots 1..4095 are Pauli-frame shots (frame_kernel).  `value` counts all 4096
+    shots over the device time of both launches; `batch64` runs 64 such circuits (Philox
+    streams 0..63) in one frame launch."""
+    from sdim_amd import workloads as W
+    from sdim_amd.engine import Job
+    job = W.repetition_code().jobs[0]
+    ops_per_shot = int((np.asarray(job.ir)[:, 0] != 0).sum())
+
+    def noiseless(j):
+        ir = np.array(j.ir).copy()
+        ir[ir[:, 0] == 17, 0] = 0
+        return Job(j.n, j.d, ir, shots=1, seed=j.seed, stream=j.stream)
+
+    def frames(j):
+        return Job(j.n, j.d, j.ir, shots=j.shots - 1, seed=j.seed, stream=j.stream,
+                   noise_channel=j.noise_channel, noise_prob=j.noise_prob)
+
+    out = {"workload": "repetition_d3_ring17", "gate_shots": ops_per_shot * job.shots}
+    tr = eng.submit([noiseless(job)], "batched").wait()
+    ref = tr.records(0)[:, 0].copy()
+    tf = eng.frame_submit([frames(job)], [ref], shot0=1).wait()
+    if warmup:
+        tr.replay(warmup)
+        tf.replay(warmup)
+    ms_ref = tr.replay(steps) / steps
+    ms_fr = tf.replay(steps) / steps
+    tr.release()
+    tf.release()
+    out.update({"value": ops_per_shot * job.shots / ((ms_ref + ms_fr) * 1e-3), "unit": "gate-shots/s",
+                "reference_shot_ms": ms_ref, "frame_ms": ms_fr, "kernel": "sdim::frame_kernel<256>"})
+    # 64 circuits in one launch
+    jobs = [Job(job.n, job.d, job.ir, shots=job.shots, seed=job.seed, stream=s,
+                noise_channel=job.noise_channel, noise_prob=job.noise_prob) for s in range(64)]
+    tr = eng.submit([noiseless(j) for j in jobs], "batched").wait()
+    refs = [tr.records(i)[:, 0].copy() for i in range(64)]
+    tf = eng.frame_submit([frames(j) for j in jobs], refs, shot0=1).wait()
+    if warmup:
+        tr.replay(warmup)
+        tf.replay(warmup)
+    ms_ref = tr.replay(steps) / steps
+    ms_fr = tf.replay(steps) / steps
+    tr.release()
+    tf.release()
+    draws = 64 * (job.shots - 1) * (job.n + int(np.isin(np.asarray(job.ir)[:, 0], (14, 15, 16, 17)).sum()))
+    insts = profiled_value("frame_kernel_repetition", "valu_insts_per_launch_batch64")
+    out["batch64"] = {"value": 64 * ops_per_shot * job.shots / ((ms_ref + ms_fr) * 1e-3), "unit": "gate-shots/s",
+                      "reference_shots_ms": ms_ref, "frame_ms": ms_fr,
+                      "philox_blocks_per_s": draws / (ms_fr * 1e-3),
+                      "roofline": None if insts is None else {
+                          "bound": "valu", "achieved": insts / (ms_fr * 1e-3), "peak": VALU_PEAK,
+                          "unit": "wave64 VALU instructions/s", "frac": round(insts / (ms_fr * 1e-3) / VALU_PEAK, 4),
+                          "traffic": None, "source": "profiles/frame_kernel_repetition.md (SQ_INSTS_VALU)"}}
+    # the frame oracle (C restatement, one core) on a bounded sample of the same frame shots
+    from oracle import c_oracle, philox
+    ir = np.asarray(job.ir)
+    noise = {int(k): (int(job.noise_channel[k]), philox.keep_threshold53(float(job.noise_prob[k])))
+             for k in np.nonzero(ir[:, 0] == 17)[0]}
+    pairs = [(int(r['measurement_value']), bool(r['deterministic'])) for r in ref]
+    n_s, t = 64, 0.0
+    while True:
+        t0 = time.perf_counter()
+        c_oracle.frame(job.n, job.d, ir, pairs, n_s, job.seed, job.stream, noise, shot0=1)
+        t = time.perf_counter() - t0
+        if t > 0.25 * cpu_seconds or n_s >= job.shots - 1:
+            break
+        n_s = min(job.shots - 1, n_s * 4)
+    out["cpu_baseline"] = {"value": ops_per_shot * n_s / t, "unit": "gate-shots/s", "cores": 1, "kind": "port",
+                           "sample": f"oracle_frame (C restatement of sdim simulate_frame) on {n_s} frame shots "
+                                     f"of the repetition-code circuit, {t:.2f} s"}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -191,6 +279,10 @@ def main():
         sys.exit(2)
     from sdim_amd.engine import Engine
 
+    if args.workload == "frame":
+        eng = Engine()
+        print(json.dumps({"frame": frame_sampler(eng, args.steps, args.warmup, args.cpu_seconds)}), flush=True)
+        return
     wl = make_workload(args.workload)
     jobs = wl.jobs
     gate_shots = wl.gate_shots
@@ -283,6 +375,9 @@ def main():
 
     if not args.no_large and args.workload == "uneven":
         out["large_tableau"] = large_tableau(eng, max(1, min(args.steps, 5)), 1)
+
+    if not args.no_frame and args.workload == "uneven":
+        out["frame"] = frame_sampler(eng, max(1, min(args.steps, 10)), 1, args.cpu_seconds)
 
     if not args.no_serve:
         out["serving"] = serving(eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)

@@ -239,6 +239,21 @@ SDIM_DEV uint32_t apply_odd(const DevOp& o, uint32_t xq, uint32_t zq, uint32_t x
     case OP_PAULI:                                 // X^a Z^b: phase += b x - a z
       dl = addm<D>(dl, subm<D>(mult<D>(S.mtab[op_b(o)], xq), mult<D>(S.mtab[op_a(o)], zq)));
       return 8;
+    case OP_CZ:                                    // z[t] += x[c], z[c] += x[t]; phase += x[c] x[t]
+      *w32(T.Z, o.t, w, rs, T.il) = addm<D>(zt, xq);
+      *pzq = addm<D>(zq, xt);
+      dl = addm<D>(dl, mulv<D>(xq, xt));
+      return 24;
+    case OP_CZI:
+      *w32(T.Z, o.t, w, rs, T.il) = subm<D>(zt, xq);
+      *pzq = subm<D>(zq, xt);
+      dl = subm<D>(dl, mulv<D>(xq, xt));
+      return 24;
+    case OP_SWAP:
+      *pxq = xt; *pzq = zt;
+      *w32(T.X, o.t, w, rs, T.il) = xq;
+      *w32(T.Z, o.t, w, rs, T.il) = zq;
+      return 32;
     case OP_NOISE: {                               // snapshot of row q for the end-of-segment pass
       uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
       ((uint32_t*)snap)[T.wofs + w] = xq;
@@ -276,6 +291,17 @@ SDIM_DEV uint32_t apply_bin(const DevOp& o, uint64_t xq, uint64_t zq, uint64_t x
     case OP_PAULI:                                 // phase += 2 (b x - a z)
       d1 ^= ((op_b(o) & 1) ? xq : 0ull) ^ ((op_a(o) & 1) ? zq : 0ull);
       return 16;
+    case OP_CZ:
+    case OP_CZI:                                   // z[t] ^= x[c], z[c] ^= x[t]; phase += 2 x[c] x[t] (mod 4)
+      *w64(T.Z, o.t, w, rs, T.il) = zt ^ xq;
+      *pzq = zq ^ xt;
+      d1 ^= xq & xt;
+      return 48;
+    case OP_SWAP:
+      *pxq = xt; *pzq = zt;
+      *w64(T.X, o.t, w, rs, T.il) = xq;
+      *w64(T.Z, o.t, w, rs, T.il) = zq;
+      return 64;
     case OP_NOISE: {
       uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
       ((uint64_t*)snap)[T.wofs + w] = xq;
@@ -286,7 +312,10 @@ SDIM_DEV uint32_t apply_bin(const DevOp& o, uint64_t xq, uint64_t zq, uint64_t x
   return 0;
 }
 
-SDIM_DEV bool two_rows(const DevOp& o) { return op_kind(o) == OP_SUM || op_kind(o) == OP_SUMI; }
+SDIM_DEV bool two_rows(const DevOp& o) {
+  const uint32_t k = op_kind(o);
+  return k == OP_SUM || k == OP_SUMI || k == OP_CZ || k == OP_CZI || k == OP_SWAP;
+}
 
 // ------------------------------------------------------------------ gate segment
 // ops[k] is the OP_SEG header.  Returns the index after the segment.

@@ -50,6 +50,30 @@ SDIM_DEV void frame_walk(const FrameParams& P, const ItemDesc& it, uint8_t* F, u
     const DevOp o = ops[k];
     const uint32_t kind = op_kind(o);
     uint8_t* fq = &F[(size_t)o.q * NT + tid];
+    if (kind >= FOP_M) {
+      // one Philox call site for every op that draws: N1 (k, shot, 0, 0), else (k, shot, 1, 1)
+      const uint64_t dom = kind == FOP_NOISE ? 0 : FRAME_DOMAIN;
+      const U64x4 r = philox4x64_10(o.src, shot, dom, dom, it.seed, it.stream);
+      const uint32_t f = *fq;
+      if (kind == FOP_NOISE) {
+        const uint64_t keep53 = ((uint64_t)(o.w0 >> 10) << 32) | o.t;
+        const uint32_t ab = noise_ab(r, d, (o.w0 >> 8) & 3u, keep53);
+        if (ab) *fq = (uint8_t)(addm(f & 15u, ab & 15u, d) | (addm(f >> 4, (ab >> 4) & 15u, d) << 4));
+        continue;
+      }
+      const uint32_t znew = (uint32_t)__umul64hi(r.v[0], (uint64_t)d) << 4;
+      const uint32_t ref = op_a(o);
+      if (kind == FOP_RESET) {
+        if (live) rec[(size_t)o.t * it.S + tid] = (uint8_t)ref;
+        *fq = (uint8_t)znew;
+        continue;
+      }
+      // M, M_X (M_X first applies the H^-1 rule); then z is drawn again
+      const uint32_t x = kind == FOP_MX ? (f >> 4) : (f & 15u);
+      if (live) rec[(size_t)o.t * it.S + tid] = (uint8_t)(addm(ref & 0x7Fu, x, d) | (ref & 0x80u));
+      *fq = (uint8_t)(x | znew);
+      continue;
+    }
     switch (kind) {
       case FOP_H: {          // (x, z) <- (-z, x)
         const uint32_t f = *fq;
@@ -98,29 +122,6 @@ SDIM_DEV void frame_walk(const FrameParams& P, const ItemDesc& it, uint8_t* F, u
         const uint8_t a = *fq, b = *ft;
         *fq = b;
         *ft = a;
-        break;
-      }
-      case FOP_M: case FOP_MX: {
-        uint32_t f = *fq;
-        if (kind == FOP_MX) f = (f >> 4) | (negm(f & 15u, d) << 4);
-        const uint32_t ref = op_a(o);
-        if (live) rec[(size_t)o.t * it.S + tid] = (uint8_t)(addm(ref & 0x7Fu, f & 15u, d) | (ref & 0x80u));
-        *fq = (uint8_t)((f & 15u) | (frame_value(it.seed, it.stream, o.src, shot, 1, d) << 4));
-        break;
-      }
-      case FOP_RESET: {
-        if (live) rec[(size_t)o.t * it.S + tid] = (uint8_t)op_a(o);
-        *fq = (uint8_t)(frame_value(it.seed, it.stream, o.src, shot, 1, d) << 4);
-        break;
-      }
-      case FOP_NOISE: {
-        const U64x4 dr = draw(it.seed, it.stream, o.src, shot);
-        const uint64_t keep53 = ((uint64_t)(o.w0 >> 10) << 32) | o.t;
-        const uint32_t ab = noise_ab(dr, d, (o.w0 >> 8) & 3u, keep53);
-        if (ab) {
-          const uint32_t f = *fq;
-          *fq = (uint8_t)(addm(f & 15u, ab & 15u, d) | (addm(f >> 4, (ab >> 4) & 15u, d) << 4));
-        }
         break;
       }
       default: break;

@@ -2,10 +2,12 @@
 // item planning, device memory and result conversion.
 //
 // Lowering restates the reference's gate decompositions for prime d
-// (sdim/tableau/tableau_gates.py): X/X_INV/Z/Z_INV become one closed-form
+// (sdim/tableau/tableau_gates.py) in closed form: X/X_INV/Z/Z_INV become one
 // Pauli conjugation (their H/P words of :27-137 net to a phase-only update),
-// CZ/CZ_INV = H^-1(t) SUM(+-1)(c,t) H(t) (:229-261), SWAP = the nine-primitive
-// prime word (:319-328), M_X = H^-1 then M (:280-296), RESET = M then
+// CZ/CZ_INV = H^-1(t) SUM(+-1)(c,t) H(t) (:229-261) become one two-row op
+// (z[t] +-= x[c], z[c] +-= x[t], phase +-= po x[c] x[t]), SWAP's nine-primitive
+// prime word (:319-328) becomes a row exchange (it changes no phase), M_X =
+// H^-1 then M (:280-296), RESET = M then
 // X^((-m) mod d) (sdim/program.py:334-339), N1 = X^a Z^b drawn with the
 // channel semantics of Program._build_ir (sdim/program.py:484-510).
 //
@@ -246,13 +248,9 @@ int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep) {
       case 8: push(OP_PI, Q, 0, 0, 0, src, 0); break;
       case 9: push(OP_SUM, Q, Tq, 0, 0, src, 0); break;
       case 10: push(OP_SUMI, Q, Tq, 0, 0, src, 0); break;
-      case 11: push(OP_HI, Tq, 0, 0, 0, src, 0); push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); break;
-      case 12: push(OP_HI, Tq, 0, 0, 0, src, 0); push(OP_SUMI, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); break;
-      case 13:
-        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
-        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
-        push(OP_SUM, Q, Tq, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0); push(OP_H, Tq, 0, 0, 0, src, 0);
-        break;
+      case 11: push(OP_CZ, Q, Tq, 0, 0, src, 0); break;
+      case 12: push(OP_CZI, Q, Tq, 0, 0, src, 0); break;
+      case 13: push(OP_SWAP, Q, Tq, 0, 0, src, 0); break;
       case 14: case 15: case 16: {
         if (gid == 15) push(OP_HI, Q, 0, 0, 0, src, 0);
         const uint32_t slot = (uint32_t)jp.meas_q.size();
@@ -359,7 +357,8 @@ void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max,
     std::vector<std::vector<uint32_t>> layers;
     for (size_t k = i; k < j; k++) {
       const Prim& p = prims[k];
-      const bool two = (p.kind == OP_SUM || p.kind == OP_SUMI);
+      const bool two = (p.kind == OP_SUM || p.kind == OP_SUMI || p.kind == OP_CZ || p.kind == OP_CZI ||
+                         p.kind == OP_SWAP);
       int32_t l = last[p.q];
       if (two) l = std::max(l, last[p.t]);
       l += 1;

@@ -108,12 +108,25 @@ SDIM_DEV uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 // ---------------------------------------------------------------- Philox4x64-10
 struct U64x4 { uint64_t v[4]; };
 
+// Full 64 x 64 -> 128-bit product from four 32 x 32 -> 64 partial products, each a
+// single multiply-add (v_mad_u64_u32).  No sum overflows: a 32-bit product is at
+// most 2^64 - 2^33 + 1 and at most two 32-bit carries are added to it.
+__host__ __device__ __forceinline__ void mul128(uint64_t a, uint64_t m, uint64_t& lo, uint64_t& hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), m0 = (uint32_t)m, m1 = (uint32_t)(m >> 32);
+  const uint64_t p00 = (uint64_t)a0 * m0;
+  const uint64_t p01 = (uint64_t)a0 * m1 + (p00 >> 32);
+  const uint64_t p10 = (uint64_t)a1 * m0 + (uint32_t)p01;
+  hi = (uint64_t)a1 * m1 + (p01 >> 32) + (p10 >> 32);
+  lo = (p10 << 32) | (uint32_t)p00;
+}
+
 SDIM_DEV U64x4 philox4x64_10(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0, uint64_t k1) {
   #pragma unroll
   for (int r = 0; r < 10; r++) {
     if (r) { k0 += 0x9E3779B97F4A7C15ull; k1 += 0xBB67AE8584CAA73Bull; }
-    uint64_t lo0 = 0xD2E7470EE14C6C93ull * c0, hi0 = __umul64hi(0xD2E7470EE14C6C93ull, c0);
-    uint64_t lo1 = 0xCA5A826395121157ull * c2, hi1 = __umul64hi(0xCA5A826395121157ull, c2);
+    uint64_t lo0, hi0, lo1, hi1;
+    mul128(c0, 0xD2E7470EE14C6C93ull, lo0, hi0);
+    mul128(c2, 0xCA5A826395121157ull, lo1, hi1);
     uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
   }

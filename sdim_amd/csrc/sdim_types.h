@@ -35,6 +35,10 @@ enum OpKind : uint32_t {
   OP_SEG = 10,      // gate segment header: q = #layers, t = #noise ops in the segment
   OP_LAYER = 11,    // layer header: q = #ops in the layer
   OP_END = 12,
+  // closed forms of the reference's words (sdim/tableau/tableau_gates.py), equal on every column:
+  OP_CZ = 13,       // H^-1(t) SUM(c,t) H(t):    z[t] += x[c], z[c] += x[t], phase += po x[c] x[t]
+  OP_CZI = 14,      // H^-1(t) SUM^-1(c,t) H(t): z[t] -= x[c], z[c] -= x[t], phase -= po x[c] x[t]
+  OP_SWAP = 15,     // the nine-primitive prime word: rows q and t exchanged, phases unchanged
 };
 
 struct DevOp {
