@@ -366,6 +366,39 @@ def test_pauli_closed_forms_match_literal_words(d):
             assert np.array_equal(t2.dph, (t.dph + sign * po * drow) % t.order)
 
 
+@pytest.mark.parametrize("d", [2, 3, 5, 7, 11])
+def test_cz_and_swap_closed_forms_match_literal_words(d):
+    """The engine's fused two-row ops: CZ^(+-1)(c, t): z[t] +-= x[c], z[c] +-= x[t],
+    ph +-= po x[c] x[t]; SWAP: rows exchanged, phases unchanged (tableau_gates.py:229-261, 319-328)."""
+    rng = np.random.default_rng(40 + d)
+    n = 5
+    po = 2 if d == 2 else 1
+    for trial in range(10):
+        t = _random_tableau(rng, n, d, depth=30)
+        t.modulo()
+        c, tg = (int(v) for v in rng.choice(n, 2, replace=False))
+        for blocks in (("x", "z", "ph"), ("dx", "dz", "dph")):
+            X, Z, P = (getattr(t, b) for b in blocks)
+            for fn, sign in ((OG.apply_CZ, 1), (OG.apply_CZ_inv, -1)):
+                t2 = t.copy()
+                fn(t2, c, tg)
+                t2.modulo()
+                ez = Z.copy()
+                ez[tg] = (Z[tg] + sign * X[c]) % d
+                ez[c] = (Z[c] + sign * X[tg]) % d
+                assert np.array_equal(getattr(t2, blocks[0]), X)
+                assert np.array_equal(getattr(t2, blocks[1]), ez)
+                assert np.array_equal(getattr(t2, blocks[2]), (P + sign * po * X[c] * X[tg]) % t.order)
+            t2 = t.copy()
+            OG.apply_SWAP(t2, c, tg)
+            t2.modulo()
+            for b, A in zip(blocks[:2], (X, Z)):
+                e = A.copy()
+                e[[c, tg]] = e[[tg, c]]
+                assert np.array_equal(getattr(t2, b), e)
+            assert np.array_equal(getattr(t2, blocks[2]), P)
+
+
 # ----------------------------------------------------------------------------- Python vs C
 @pytest.mark.parametrize("d", [2, 3, 5, 7, 11])
 def test_python_oracle_matches_c_restatement(d):
