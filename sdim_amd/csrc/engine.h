@@ -48,8 +48,10 @@ __host__ __device__ inline uint32_t noise_bytes(uint32_t nn, uint32_t rs, uint32
 
 // ---- gangs: one item processed by K workgroups, each owning a range of column words
 struct GangSync;
+// bytes of one gang's barrier record (GangSync in engine.hip)
+constexpr uint64_t GANG_SYNC_BYTES = 4096;
 struct GangParams {
-  GangSync* sync;       // one 256-byte record per gang
+  GangSync* sync;       // one GANG_SYNC_BYTES record per gang
   uint8_t* scr;         // per gang: [n_max][Kp] + [S_max][Kp] bytes, acc[K], crs[K], tot[S_max] u32,
                         // two pivot-column staging buffers
   uint64_t scr_stride;  // bytes of one gang's scratch

@@ -1236,13 +1236,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
 // member that waits past a deadline raises the gang's error word, and every
 // member leaves the kernel when it sees it.
 struct GangSync {
-  uint32_t ctr;        // barrier arrivals
+  uint32_t ctr;        // barrier arrivals (one per XCC group after the first barrier)
   int err;             // nonzero: a member timed out
   uint32_t item[2];    // item index broadcast (double-buffered)
   int piv[2];          // pivot candidates (double-buffered by measurement count)
-  uint32_t pad[58];
+  uint32_t xcnt[16];   // members on each XCC (counted once, before the first barrier)
+  uint32_t pad[42];
+  struct { uint32_t v; uint32_t pad[31]; } xarr[16];   // per-XCC arrivals, one 128-byte line each
+  uint32_t tail[448];
 };
-static_assert(sizeof(GangSync) == 256, "GangSync is 256 bytes");
+static_assert(sizeof(GangSync) == GANG_SYNC_BYTES, "GangSync size");
 
 constexpr uint64_t GANG_TIMEOUT = 1000000000ull;   // 10 s of the 100 MHz constant clock
 
@@ -1257,16 +1260,40 @@ struct GangCtx {
   uint32_t K, Kp, me;
   int* gerr;
   uint32_t target;     // arrivals expected at the next barrier
+  uint32_t xcc;        // this member's XCC (shared L2)
+  uint32_t xc, ngrp;   // members on this XCC, XCCs holding members (0 until counted)
+  uint32_t xtarget;    // arrivals expected at this XCC's next barrier
 };
 
+// XCC (accelerator die, one L2 each) of the executing wave
+SDIM_DEV uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 15u;
+}
+
+// Barrier of a gang.  The first one (g.ngrp == 0) counts every member and releases each
+// member's writes on its own.  Later ones are two-level: members on one XCC share an L2,
+// so each member only waits until its own stores are acknowledged by that L2 and counts
+// itself in its XCC's record; the XCC's last arriver releases the L2 (one write-back for
+// the whole group) and counts the group once; everyone polls the group count.
 SDIM_DEV bool gang_barrier(GangCtx& g, int* s_ok) {
+  const bool grouped = g.ngrp != 0;
+  if (grouped) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's stores are in L2
   __syncthreads();
-  g.target += g.K;
+  g.target += grouped ? g.ngrp : g.K;
+  if (grouped) g.xtarget += g.xc;
   if (threadIdx.x == 0) {
     int ok = 1;
-    // release: this workgroup's writes (ordered before thread 0 by the barrier above) become
-    // visible device-wide before the arrival is counted
-    __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (grouped) {
+      const uint32_t old = __hip_atomic_fetch_add(&g.sy->xarr[g.xcc].v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == g.xtarget - 1)   // last of the XCC: write back its L2, then count the group
+        __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      // release: this workgroup's writes (ordered before thread 0 by the barrier above) become
+      // visible device-wide before the arrival is counted
+      __hip_atomic_fetch_add(&g.sy->ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
     const uint64_t t0 = wall_clock64();
     while (__hip_atomic_load(&g.sy->ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g.target) {
       if (__hip_atomic_load(&g.sy->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
@@ -1803,6 +1830,21 @@ __global__ __launch_bounds__(NT) void gang_kernel(KParams P, GangParams G) {
   Gc.me = blockIdx.x % G.K;
   Gc.gerr = G.err;
   Gc.target = 0;
+  Gc.xcc = xcc_id();
+  Gc.xc = 0; Gc.ngrp = 0; Gc.xtarget = 0;
+  // members per XCC: counted before the first (flat) barrier, read after it
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&Gc.sy->xcnt[Gc.xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!gang_barrier(Gc, &s_ok)) return;
+  {
+    uint32_t xc = 0, ng = 0;
+    for (uint32_t i = 0; i < 16; i++) {
+      const uint32_t c = __hip_atomic_load(&Gc.sy->xcnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ng += c ? 1u : 0u;
+      if (i == Gc.xcc) xc = c;
+    }
+    Gc.xc = xc;
+    Gc.ngrp = ng;
+  }
   uint32_t mk = 0, round = 0;
   for (;;) {
     if (Gc.me == 0 && threadIdx.x == 0)
@@ -1824,6 +1866,7 @@ __global__ void gang_reset_kernel(GangSync* sync, uint32_t n_gangs, int* err) {
     sync[i].err = 0;
     sync[i].item[0] = 0; sync[i].item[1] = 0;
     sync[i].piv[0] = INT_MAX; sync[i].piv[1] = INT_MAX;
+    for (int x = 0; x < 16; x++) { sync[i].xcnt[x] = 0; sync[i].xarr[x].v = 0; }
   }
   if (threadIdx.x == 0) *err = 0;
 }

@@ -962,7 +962,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     g.gp.err = (int*)(uintptr_t)gang_bytes;
     gang_bytes += 256;
     g.gp.sync = (GangSync*)(uintptr_t)gang_bytes;
-    gang_bytes += 256ull * n_gangs;
+    gang_bytes += GANG_SYNC_BYTES * n_gangs;
     g.gp.scr = (uint8_t*)(uintptr_t)gang_bytes;
     gang_bytes += g.gp.scr_stride * n_gangs;
   }
