@@ -125,7 +125,11 @@ SDIM_DEV uint64_t* plane(const Tab& T, uint32_t s, int k) { return (uint64_t*)(T
 
 // Optional per-phase timers (build with -DSDIM_PHASE_TIMERS): thread 0 of each
 // workgroup accumulates wall-clock ticks per phase into counters[2 + phase].
+// Switched off: a timer build of batch_kernel<1024> stopped with a memory-aperture
+// violation on the uneven batch (the cause is not known; the kernel spills far more
+// registers in that build), so the timers stay out until the cause is found.
 #ifdef SDIM_PHASE_TIMERS
+#error "SDIM_PHASE_TIMERS is switched off (see the comment above)"
 #define SDIM_T0() const uint64_t t_start_ = (threadIdx.x == 0) ? wall_clock64() : 0
 #define SDIM_TMARK(var) uint64_t var = (threadIdx.x == 0) ? wall_clock64() : 0
 #define SDIM_TADD(P, ph, from) do { if (threadIdx.x == 0) atomicAdd(&(P).counters[2 + (ph)], (unsigned long long)(wall_clock64() - (from))); } while (0)
