@@ -23,6 +23,8 @@ Besides the headline (batched persistent path) the line carries:
   cpu_baseline   the CPU oracle (literal restatement of the reference's
                  Program.simulate tableau path, same interpreted cost structure),
                  timed on a bounded sample on this host, 1 core
+  cpu_baseline_parallel  the same oracle in one process per host core (up to 16),
+                 circuits dealt round-robin, timed before the GPU is touched
 """
 import argparse
 import json
@@ -131,6 +133,50 @@ def cpu_baseline(jobs, budget_s):
             "sample": f"oracle/program.py (literal restatement of sdim Program.simulate tableau path, "
                       f"interpreted per-gate loops) on the first {done_jobs} circuit(s) of the workload "
                       f"(n={ns}), 1 shot each, {done_ops} gate-shots in {t_total:.1f} s",
+            "host_cores_available": len(os.sched_getaffinity(0))}
+
+
+def _cpu_worker(task):
+    """One process of the parallel CPU baseline: the literal oracle on its circuits, 1 shot each,
+    until its time budget is used (the circuit in progress is finished)."""
+    items, budget_s = task
+    from oracle import program as OP
+    ops_done, t_total = 0, 0.0
+    for n, d, ir, seed, stream, noise in items:
+        t0 = time.perf_counter()
+        OP.simulate(n, d, [tuple(r) for r in ir], shots=1, seed=seed, stream=stream, noise=noise, keep_tableaux=False)
+        t_total += time.perf_counter() - t0
+        ops_done += sum(1 for r in ir if r[0] != 0)
+        if t_total >= budget_s:
+            break
+    return ops_done, t_total
+
+
+def cpu_baseline_parallel(jobs, budget_s, procs):
+    """The same literal oracle in `procs` processes on the host cores, circuits dealt round-robin
+    (circuits and shots are independent, so the reference's work parallelises this way).  Runs
+    before this process touches the GPU: the workers are started with the spawn method."""
+    import multiprocessing as mp
+    from oracle import philox
+    tasks = [[] for _ in range(procs)]
+    for i, j in enumerate(jobs[:procs * 8]):
+        ir = np.asarray(j.ir).reshape(-1, 3)
+        noise = {}
+        if j.noise_channel is not None:
+            for k in np.nonzero(ir[:, 0] == 17)[0]:
+                noise[int(k)] = (int(j.noise_channel[k]), philox.keep_threshold53(float(j.noise_prob[k])))
+        tasks[i % procs].append((int(j.n), int(j.d), ir.tolist(), int(j.seed), int(j.stream), noise))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [(t, budget_s) for t in tasks])
+    wall = time.perf_counter() - t0
+    ops = sum(r[0] for r in res)
+    # the processes run concurrently: the aggregate rate is the sum of their rates
+    rate = sum(r[0] / r[1] for r in res if r[1] > 0)
+    return {"value": rate, "unit": "gate-shots/s", "cores": procs, "kind": "port",
+            "sample": f"oracle/program.py in {procs} concurrent processes, circuits of the workload dealt "
+                      f"round-robin, 1 shot each; {ops} gate-shots, each process timed over its own circuits "
+                      f"({wall:.1f} s wall), rates summed",
             "host_cores_available": len(os.sched_getaffinity(0))}
 
 
@@ -277,6 +323,11 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
                                    "(sharding is described in DESIGN.md, not built)"}))
         sys.exit(2)
+    cpu_par = None
+    if not args.no_cpu and args.workload != "frame":
+        # before anything touches the GPU (the worker processes are spawned from this one)
+        procs = min(16, len(os.sched_getaffinity(0)))
+        cpu_par = cpu_baseline_parallel(make_workload(args.workload).jobs, args.cpu_seconds / 2, procs)
     from sdim_amd.engine import Engine
 
     if args.workload == "frame":
@@ -384,6 +435,7 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_seconds)
+        out["cpu_baseline_parallel"] = cpu_par
     print(json.dumps(out), flush=True)
 
 

@@ -58,6 +58,9 @@ namespace sdim {
 #ifndef SDIM_BALLOT_TR
 #define SDIM_BALLOT_TR 0
 #endif
+#ifndef SDIM_SWEEP_RB
+#define SDIM_SWEEP_RB 2   // rows in flight per thread in the one-workgroup sweep
+#endif
 #ifndef SDIM_DET_BATCH
 #define SDIM_DET_BATCH 1
 #endif
@@ -797,7 +800,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
                       const Scr& S, const ItemDesc& it, const KParams& P, unsigned long long& bytes) {
   constexpr bool BIN = (D == 2);
   constexpr uint32_t CPW = BIN ? 64 : 4;
-  constexpr int RB = 2;     // rows per load batch in the sweep (16 bytes of X and Z each)
+  constexpr int RB = SDIM_SWEEP_RB;   // rows per load batch in the sweep (16 bytes of X and Z each)
   const uint32_t tid = threadIdx.x;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t G = 2 * n;
