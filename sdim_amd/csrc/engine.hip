@@ -49,7 +49,9 @@
 namespace sdim {
 
 // Measurement variants (compile-time switches, for measuring their effect):
-//   SDIM_EARLY_DRAWS  draw random outcomes during step A instead of step E
+//   SDIM_EARLY_DRAWS  for items whose tableau is in HBM, draw random outcomes during step A
+//                     (hiding the draws behind the loads of row q) instead of step E; items in
+//                     LDS always draw in step E, and only when the outcome is random
 //   SDIM_BALLOT_TR    compact the touched-row list with wave ballots instead of per-row LDS atomics
 //   SDIM_DET_BATCH    words per load batch in the deterministic scan
 #ifndef SDIM_EARLY_DRAWS
@@ -813,7 +815,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 
   // random outcomes do not depend on the tableau: draw them first (step E uses them when
   // the measurement turns out to be random)
-  if constexpr (SDIM_EARLY_DRAWS)
+  const bool early = SDIM_EARLY_DRAWS && !(it.flags & ITEM_LDS);
+  if (early)
     for (uint32_t s = tid; s < Sh; s += NT) S.shot[Sh + s] = outcome_of(draw(it.seed, it.stream, src, it.shot0 + s), D);
   // ---- A: row q -> pivot candidate, f, groups with f != 0, copy of row q
   rowq_groups<D, NT>(T, 0, q, S, M, S.RQ, bytes);
@@ -963,7 +966,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     // ---- E: outcome per shot (drawn in step A); ph_{n+p} <- ph_p, ph_p <- -m po
     for (uint32_t s = tid; s < Sh; s += NT) {
       uint32_t m;
-      if constexpr (SDIM_EARLY_DRAWS) m = S.shot[Sh + s];
+      if (early) m = S.shot[Sh + s];
       else { m = outcome_of(draw(it.seed, it.stream, src, it.shot0 + s), D); S.shot[Sh + s] = m; }
       if constexpr (BIN) {
         const uint32_t k2 = S.shot[s];
@@ -1536,42 +1539,49 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     SDIM_TADD(P, 5, tC);
     SDIM_TMARK(tD);
 
-    // ---- D: own parts of columns p and n+p; per-shot phases over own words with f != 0
-    if (own_p || own_g) {
+    // ---- D: own parts of columns p and n+p; per-shot phases over own words with f != 0.
+    // Column n+p is rewritten at every row (the old destabiliser's support is not known);
+    // column p only at the touched rows: elsewhere it is the pivot column's zero already.
+    if (own_g || (own_p && BIN && (p >> 6) == ((n + p) >> 6))) {
       for (uint32_t r = tid; r < n; r += NT) {
         if constexpr (BIN) {
           const uint64_t xv = S.xp[r], zv = S.zp[r];
           const uint32_t g = n + p;
-          if (own_p) {
+          if ((p >> 6) == (g >> 6)) {        // p and n+p share a word: both bits at every row
             uint64_t* xw = w64(V.X, r, (p >> 6) - w0, rs, V.il);
             uint64_t* zw = w64(V.Z, r, (p >> 6) - w0, rs, V.il);
             uint64_t x = *xw & ~(1ull << (p & 63));
             uint64_t z = (*zw & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-            if ((g >> 6) == (p >> 6)) {
-              x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
-              z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
-            }
+            x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
+            z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
             *xw = x; *zw = z;
-          }
-          if (own_g && (g >> 6) != (p >> 6)) {
+          } else {
             uint64_t* xw = w64(V.X, r, (g >> 6) - w0, rs, V.il);
             uint64_t* zw = w64(V.Z, r, (g >> 6) - w0, rs, V.il);
             *xw = (*xw & ~(1ull << (g & 63))) | (xv << (g & 63));
             *zw = (*zw & ~(1ull << (g & 63))) | (zv << (g & 63));
           }
-          if (own_p) bytes += 16;
-          if (own_g && (g >> 6) != (p >> 6)) bytes += 16;
+          bytes += 16;
         } else {
-          if (own_p) {
-            *b8(V.X, r, p - c_lo, rs, V.il) = 0;
-            *b8(V.Z, r, p - c_lo, rs, V.il) = (uint8_t)(r == q);
-            bytes += 2;
-          }
-          if (own_g) {
-            *b8(V.X, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.xp[r]) % D);
-            *b8(V.Z, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.zp[r]) % D);
-            bytes += 2;
-          }
+          *b8(V.X, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.xp[r]) % D);
+          *b8(V.Z, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.zp[r]) % D);
+          bytes += 2;
+        }
+      }
+    }
+    if (own_p && !(BIN && (p >> 6) == ((n + p) >> 6))) {
+      for (uint32_t k = tid; k < nT; k += NT) {
+        const uint32_t r = S.TR[k];
+        if constexpr (BIN) {
+          uint64_t* xw = w64(V.X, r, (p >> 6) - w0, rs, V.il);
+          uint64_t* zw = w64(V.Z, r, (p >> 6) - w0, rs, V.il);
+          *xw &= ~(1ull << (p & 63));
+          *zw = (*zw & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
+          bytes += 16;
+        } else {
+          *b8(V.X, r, p - c_lo, rs, V.il) = 0;
+          *b8(V.Z, r, p - c_lo, rs, V.il) = (uint8_t)(r == q);
+          bytes += 2;
         }
       }
     }

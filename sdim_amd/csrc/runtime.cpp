@@ -1505,7 +1505,7 @@ void serve_loop(sdim_server* s) {
     // ---- admission: FIFO while the footprint fits the budget
     std::vector<uint64_t> take;
     uint64_t take_bytes = 0;
-    bool quit = false;
+    bool quit = false, rejected = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       if (s->stop && s->pending.empty() && s->inflight.empty()) quit = true;
@@ -1516,6 +1516,7 @@ void serve_loop(sdim_server* s) {
             r.state = 3;
             r.error = "request larger than the HBM budget";
             s->pending.pop_front();
+            rejected = true;
             continue;
           }
           if (s->used + take_bytes + r.footprint > s->budget) break;
@@ -1525,6 +1526,7 @@ void serve_loop(sdim_server* s) {
         }
         s->used += take_bytes;
       }
+      if (rejected) s->cv.notify_all();
       if (!quit && take.empty()) {
         // nothing to launch: wait for an arrival or poll completions again shortly
         s->cv.wait_for(lk, std::chrono::microseconds(s->inflight.empty() ? 2000 : 50));
@@ -1634,7 +1636,8 @@ int sdim_server_wait(sdim_server* s, uint64_t id, double timeout_s) {
     if (it == s->reqs.end()) return fail(SDIM_ENOTFOUND, "unknown request");
     if (it->second.state == 2) return 0;
     if (it->second.state == 3) return fail(SDIM_EINVAL, it->second.error);
-    if (s->cv.wait_until(lk, deadline) == std::cv_status::timeout) return fail(SDIM_EBUSY, "timeout");
+    if (std::chrono::steady_clock::now() >= deadline) return fail(SDIM_EBUSY, "timeout");
+    s->cv.wait_until(lk, deadline);   // the state is checked again after every wake-up
   }
 }
 
