@@ -242,9 +242,10 @@ def large_tableau(eng, steps, warmup):
 def frame_sampler(eng, steps, warmup, cpu_seconds):
     """Repetition-code configuration (d=3, 17+17 qutrits, 10 rounds, p=1e-3, 4096 shots) through
     the reference's default shots > 1 path (sdim/program.py:244-265): shot 0 is a noiseless
-    tableau shot, shots 1..4095 are Pauli-frame shots (frame_kernel).  `value` counts all 4096
-    shots over the device time of both launches; `batch64` runs 64 such circuits (Philox
-    streams 0..63) in one frame launch."""
+    tableau shot, shots 1..4095 are Pauli-frame shots (frame_kernel), both in one submission
+    (the reference shot runs concurrently with the frames).  `value` counts all 4096 shots over
+    that submission's device time; the two launches are also timed on their own; `batch64`
+    runs 64 such circuits (Philox streams 0..63) per submission."""
     from sdim_amd import workloads as W
     from sdim_amd.engine import Job
     job = W.repetition_code().jobs[0]
@@ -260,35 +261,43 @@ def frame_sampler(eng, steps, warmup, cpu_seconds):
                    noise_channel=j.noise_channel, noise_prob=j.noise_prob)
 
     out = {"workload": "repetition_d3_ring17", "gate_shots": ops_per_shot * job.shots}
+
+    def timed(tk):
+        if warmup:
+            tk.replay(warmup)
+        ms = tk.replay(steps) / steps
+        return ms
+
+    # the two launches on their own: the reference shot (tableau) and the frame shots
     tr = eng.submit([noiseless(job)], "batched").wait()
     ref = tr.records(0)[:, 0].copy()
     tf = eng.frame_submit([frames(job)], [ref], shot0=1).wait()
-    if warmup:
-        tr.replay(warmup)
-        tf.replay(warmup)
-    ms_ref = tr.replay(steps) / steps
-    ms_fr = tf.replay(steps) / steps
+    ms_ref, ms_fr = timed(tr), timed(tf)
     tr.release()
     tf.release()
-    out.update({"value": ops_per_shot * job.shots / ((ms_ref + ms_fr) * 1e-3), "unit": "gate-shots/s",
-                "reference_shot_ms": ms_ref, "frame_ms": ms_fr, "kernel": "sdim::frame_kernel<256>"})
-    # 64 circuits in one launch
+    # one submission: the library runs the reference shot concurrently with the frames
+    tc = eng.frame_submit([job]).wait()
+    ms_c = timed(tc)
+    tc.release()
+    out.update({"value": ops_per_shot * job.shots / (ms_c * 1e-3), "unit": "gate-shots/s", "device_ms": ms_c,
+                "reference_shot_ms": ms_ref, "frame_ms": ms_fr,
+                "kernels": "sdim::batch_kernel (reference shot) || sdim::frame_kernel<256> (frame shots)"})
+    # 64 circuits in one submission
     jobs = [Job(job.n, job.d, job.ir, shots=job.shots, seed=job.seed, stream=s,
                 noise_channel=job.noise_channel, noise_prob=job.noise_prob) for s in range(64)]
     tr = eng.submit([noiseless(j) for j in jobs], "batched").wait()
     refs = [tr.records(i)[:, 0].copy() for i in range(64)]
     tf = eng.frame_submit([frames(j) for j in jobs], refs, shot0=1).wait()
-    if warmup:
-        tr.replay(warmup)
-        tf.replay(warmup)
-    ms_ref = tr.replay(steps) / steps
-    ms_fr = tf.replay(steps) / steps
+    ms_ref, ms_fr = timed(tr), timed(tf)
     tr.release()
     tf.release()
+    tc = eng.frame_submit(jobs).wait()
+    ms_c = timed(tc)
+    tc.release()
     draws = 64 * (job.shots - 1) * (job.n + int(np.isin(np.asarray(job.ir)[:, 0], (14, 15, 16, 17)).sum()))
     insts = profiled_value("frame_kernel_repetition", "valu_insts_per_launch_batch64")
-    out["batch64"] = {"value": 64 * ops_per_shot * job.shots / ((ms_ref + ms_fr) * 1e-3), "unit": "gate-shots/s",
-                      "reference_shots_ms": ms_ref, "frame_ms": ms_fr,
+    out["batch64"] = {"value": 64 * ops_per_shot * job.shots / (ms_c * 1e-3), "unit": "gate-shots/s",
+                      "device_ms": ms_c, "reference_shots_ms": ms_ref, "frame_ms": ms_fr,
                       "philox_blocks_per_s": draws / (ms_fr * 1e-3),
                       "roofline": None if insts is None else {
                           "bound": "valu", "achieved": insts / (ms_fr * 1e-3), "peak": VALU_PEAK,

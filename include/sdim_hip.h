@@ -118,6 +118,11 @@ int sdim_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, int32_t mod
  * measurement at instruction k (k, shot, 1, 1), N1 (k, shot, 0, 0) as in the tableau
  * path.  Jobs start from |0...0> (init must be NULL).  Records are fetched with
  * sdim_num_measurements / sdim_fetch_records; such a ticket keeps no tableaux.
+ * refs == NULL (shot0 must be 0): the library simulates each job's reference shot itself as
+ * shot 0 (a tableau run with N1 as the identity, sdim/program.py:244-247, concurrently with
+ * the frames on another stream) and the job's `shots` counts it: records are the full
+ * [event][shot] table of Program.simulate's frame branch, and sdim_fetch_tableau(shot 0)
+ * returns the reference shot's final tableau.
  */
 int sdim_frame_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
                       uint64_t* ticket);

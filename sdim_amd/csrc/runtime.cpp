@@ -68,6 +68,7 @@ struct JobPlan {
   int64_t shot_base = 0;     // shot index of the job's first shot (frame sampling starts after the reference shot)
   int64_t n_meas = 0;
   std::vector<int32_t> meas_q, meas_round;
+  std::vector<uint8_t> meas_kind;   // frame tickets: gate id of each event (14 M, 15 M_X, 16 RESET)
   uint32_t S_item = 1;
   bool resident = false;
   uint32_t first_item = 0, n_items = 0;
@@ -146,6 +147,10 @@ struct Ticket {
   // frame-sampler tickets
   FrameParams FP{};
   uint32_t frame_lds = 0;
+  // frame tickets that simulate their own reference shots: the tableau ticket of those shots
+  // (shot 0 of every job, run concurrently on another stream); frame records then hold the
+  // frames' x components only and are combined with the reference records when fetched
+  std::unique_ptr<Ticket> child;
 };
 
 // ticket mode of frame-sampler submissions (sdim_frame_submit)
@@ -294,6 +299,7 @@ int frame_expand(const sdim_job& j, const uint8_t* ref, JobPlan& jp, std::vector
   std::vector<int32_t> counts((size_t)n, 0);
   jp.meas_q.clear();
   jp.meas_round.clear();
+  jp.meas_kind.clear();
   for (int64_t k = 0; k < j.n_ops; k++) {
     const int64_t gid = j.ir[3 * k], q = j.ir[3 * k + 1], t = j.ir[3 * k + 2];
     if (gid < 0 || gid > 17) return fail(SDIM_EINVAL, "Invalid gate value");
@@ -304,19 +310,22 @@ int frame_expand(const sdim_job& j, const uint8_t* ref, JobPlan& jp, std::vector
     if (gid <= 4) continue;                       // I, X, Z and inverses leave a frame unchanged
     if (gid >= 14 && gid <= 16) {
       const int64_t e = (int64_t)jp.meas_q.size();
-      const uint8_t* r = ref + 33 * (size_t)e;
-      int64_t rq, rround, rval;
-      memcpy(&rq, r, 8);
-      memcpy(&rround, r + 8, 8);
-      memcpy(&rval, r + 25, 8);
-      if (rq != q || rround != counts[(size_t)q] || rval < 0 || rval >= (int64_t)d || r[24] > 1)
-        return fail(SDIM_EINVAL, "reference record " + std::to_string(e) + " does not match measurement op " +
-                                     std::to_string(k) + " (qudit " + std::to_string(q) + ", round " +
-                                     std::to_string(counts[(size_t)q]) + ")");
-      o.w0 |= (uint32_t)(rval | (r[24] << 7)) << 8;
+      if (ref) {   // without reference records the kernel writes the frames' x components only
+        const uint8_t* r = ref + 33 * (size_t)e;
+        int64_t rq, rround, rval;
+        memcpy(&rq, r, 8);
+        memcpy(&rround, r + 8, 8);
+        memcpy(&rval, r + 25, 8);
+        if (rq != q || rround != counts[(size_t)q] || rval < 0 || rval >= (int64_t)d || r[24] > 1)
+          return fail(SDIM_EINVAL, "reference record " + std::to_string(e) + " does not match measurement op " +
+                                       std::to_string(k) + " (qudit " + std::to_string(q) + ", round " +
+                                       std::to_string(counts[(size_t)q]) + ")");
+        o.w0 |= (uint32_t)(rval | (r[24] << 7)) << 8;
+      }
       o.t = (uint32_t)e;
       jp.meas_q.push_back((int32_t)q);
       jp.meas_round.push_back(counts[(size_t)q]++);
+      jp.meas_kind.push_back((uint8_t)gid);
     } else if (gid == 17) {
       if (!j.noise_channel || !j.noise_prob) return fail(SDIM_EINVAL, "N1 op without noise parameters");
       const int32_t ch = j.noise_channel[k];
@@ -480,8 +489,11 @@ Ticket* find_ticket(sdim_ctx* c, uint64_t id) {
 }
 
 int finish(sdim_ctx* c, Ticket* t) {
-  (void)c;
   if (t->finished) return 0;
+  if (t->child) {
+    const int rc = finish(c, t->child.get());
+    if (rc) return rc;
+  }
   HIP_OK(hipStreamSynchronize(t->stream));
   t->finished = true;
   for (int e : t->h_gang_err)
@@ -489,16 +501,23 @@ int finish(sdim_ctx* c, Ticket* t) {
   return 0;
 }
 
-// frees a ticket's device memory and graphs and forgets it
-void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::iterator it) {
-  Ticket* t = it->second.get();
+// frees a ticket's (and its child's) device memory, graphs and events
+void release_ticket(sdim_ctx* c, Ticket* t) {
+  if (t->child) release_ticket(c, t->child.get());
   ctx_free(c, t->arena);
   ctx_free(c, t->meta);
   ctx_free(c, t->gang);
-  for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
+  for (auto& g : t->graph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
+  t->ev0 = t->ev1 = nullptr;
   for (auto e : t->fork_join) (void)hipEventDestroy(e);
+  t->fork_join.clear();
+}
+
+// releases a ticket and forgets it
+void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::iterator it) {
+  release_ticket(c, it->second.get());
   c->tickets.erase(it);
 }
 
@@ -633,15 +652,16 @@ int sdim_ctx_destroy(sdim_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tickets) {
-    Ticket* t = kv.second.get();
-    (void)hipStreamSynchronize(t->stream);
-    if (t->arena.p) (void)hipFree(t->arena.p);
-    if (t->meta.p) (void)hipFree(t->meta.p);
-    if (t->gang.p) (void)hipFree(t->gang.p);
-    for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
-    if (t->ev0) (void)hipEventDestroy(t->ev0);
-    if (t->ev1) (void)hipEventDestroy(t->ev1);
-    for (auto e : t->fork_join) (void)hipEventDestroy(e);
+    for (Ticket* t = kv.second.get(); t; t = t->child.get()) {
+      (void)hipStreamSynchronize(t->stream);
+      if (t->arena.p) (void)hipFree(t->arena.p);
+      if (t->meta.p) (void)hipFree(t->meta.p);
+      if (t->gang.p) (void)hipFree(t->gang.p);
+      for (auto& g : t->graph) if (g) (void)hipGraphExecDestroy(g);
+      if (t->ev0) (void)hipEventDestroy(t->ev0);
+      if (t->ev1) (void)hipEventDestroy(t->ev1);
+      for (auto e : t->fork_join) (void)hipEventDestroy(e);
+    }
   }
   for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
   for (auto st : c->streams) (void)hipStreamDestroy(st);
@@ -1060,14 +1080,59 @@ int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
 int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
                       uint64_t* ticket) {
   if (!c) return fail(SDIM_EINVAL, "null context");
-  if ((!jobs && n_jobs) || n_jobs < 0 || !ticket || (n_jobs && !refs)) return fail(SDIM_EINVAL, "bad submit arguments");
+  if ((!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
   if (shot0 < 0) return fail(SDIM_EINVAL, "shot0 must be >= 0");
+  const bool own_ref = refs == nullptr;
+  if (own_ref && shot0 != 0) return fail(SDIM_EINVAL, "without reference records shot0 must be 0 (shot 0 is the reference shot)");
   std::lock_guard<std::mutex> lk(c->mu);
   HIP_OK(hipSetDevice(c->device));
   std::unique_ptr<Ticket> T(new Ticket());
   T->mode = MODE_FRAME;
   T->stream = c->stream;
   T->jobs.resize((size_t)n_jobs);
+  if (own_ref) {
+    // the reference shots: each job once, N1 as the identity (sdim/program.py:31, 244-247),
+    // as a tableau submission on another stream, overlapping the frame launch
+    for (int32_t ji = 0; ji < n_jobs; ji++) {
+      const sdim_job& j = jobs[ji];
+      if (j.n_ops < 0 || (j.n_ops && !j.ir)) return fail(SDIM_EINVAL, "bad op list");
+      if (j.init[0] || j.init[1] || j.init[2] || j.init[3] || j.init[4] || j.init[5])
+        return fail(SDIM_EINVAL, "the frame sampler starts from |0...0> (its frames start as random Z)");
+    }
+    std::vector<std::vector<int64_t>> irs((size_t)n_jobs);
+    std::vector<sdim_job> rj((size_t)n_jobs);
+    for (int32_t ji = 0; ji < n_jobs; ji++) {
+      const sdim_job& j = jobs[ji];
+      irs[(size_t)ji].assign(j.ir, j.ir + 3 * j.n_ops);
+      for (int64_t k = 0; k < j.n_ops; k++) if (irs[(size_t)ji][3 * k] == 17) irs[(size_t)ji][3 * k] = 0;
+      sdim_job r = j;
+      r.ir = irs[(size_t)ji].data();
+      r.noise_channel = nullptr;
+      r.noise_prob = nullptr;
+      r.shots = 1;
+      rj[(size_t)ji] = r;
+    }
+    uint64_t rid = 0;
+    int rc = submit_locked(c, rj.data(), n_jobs, SDIM_MODE_BATCHED, 1, &rid, nullptr);
+    if (rc) return rc;
+    auto it = c->tickets.find(rid);
+    T->child = std::move(it->second);
+    c->tickets.erase(it);
+  }
+  // errors from here on: wait for the reference shots in flight and free them
+  auto bail = [&](int code) {
+    if (T->child) {
+      (void)hipStreamSynchronize(T->child->stream);
+      release_ticket(c, T->child.get());
+      T->child.reset();
+    }
+    return code;
+  };
+#define FRAME_HIP_OK(call)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return bail(fail(SDIM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+  } while (0)
   std::vector<DevOp> ops;
   std::vector<ItemDesc>& items = T->items;
   const uint32_t NT = FRAME_NT;
@@ -1076,33 +1141,35 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
   for (int32_t ji = 0; ji < n_jobs; ji++) {
     const sdim_job& j = jobs[ji];
     JobPlan& jp = T->jobs[(size_t)ji];
-    if (j.n < 1) return fail(SDIM_EINVAL, "Number of qudits must be greater than 0");
-    if (j.d < 2) return fail(SDIM_EINVAL, "Dimension must be greater than 1");
-    if (!supported_dim(j.d)) return fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})");
-    if (j.shots < 1) return fail(SDIM_EINVAL, "shots must be >= 1");
-    if (shot0 + j.shots > 0xFFFFFFFFll) return fail(SDIM_EINVAL, "shot indices must stay below 2^32");
-    if (j.n > 32768) return fail(SDIM_EINVAL, "at most 32768 qudits are supported");
-    if (j.n_ops < 0 || (j.n_ops && !j.ir)) return fail(SDIM_EINVAL, "bad op list");
+    if (j.n < 1) return bail(fail(SDIM_EINVAL, "Number of qudits must be greater than 0"));
+    if (j.d < 2) return bail(fail(SDIM_EINVAL, "Dimension must be greater than 1"));
+    if (!supported_dim(j.d)) return bail(fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})"));
+    if (j.shots < 1) return bail(fail(SDIM_EINVAL, "shots must be >= 1"));
+    if (shot0 + j.shots > 0xFFFFFFFFll) return bail(fail(SDIM_EINVAL, "shot indices must stay below 2^32"));
+    if (j.n > 32768) return bail(fail(SDIM_EINVAL, "at most 32768 qudits are supported"));
+    if (j.n_ops < 0 || (j.n_ops && !j.ir)) return bail(fail(SDIM_EINVAL, "bad op list"));
     if (j.init[0] || j.init[1] || j.init[2] || j.init[3] || j.init[4] || j.init[5])
-      return fail(SDIM_EINVAL, "the frame sampler starts from |0...0> (its frames start as random Z)");
+      return bail(fail(SDIM_EINVAL, "the frame sampler starts from |0...0> (its frames start as random Z)"));
     jp.n = j.n;
     jp.d = j.d;
     jp.shots = j.shots;
     jp.shot_base = shot0;
     jp.ops_off = (uint32_t)ops.size();
-    const uint8_t* ref = (const uint8_t*)refs[ji];
+    const uint8_t* ref = own_ref ? nullptr : (const uint8_t*)refs[ji];
     // events are counted before the records are read
     int64_t n_ev = 0;
     for (int64_t k = 0; k < j.n_ops; k++) n_ev += (j.ir[3 * k] >= 14 && j.ir[3 * k] <= 16);
-    if (n_ev && !ref) return fail(SDIM_EINVAL, "missing reference records");
+    if (n_ev && !own_ref && !ref) return bail(fail(SDIM_EINVAL, "missing reference records"));
     int rc = frame_expand(j, ref, jp, ops);
-    if (rc) return rc;
+    if (rc) return bail(rc);
     jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
     const uint32_t n = (uint32_t)j.n;
     const bool in_lds = (uint64_t)n * NT <= (uint64_t)c->frame_lds_bytes;
     if (in_lds) lds = std::max<uint32_t>(lds, (n * NT + 15u) & ~15u);
     jp.first_item = (uint32_t)items.size();
-    for (int64_t s0 = 0; s0 < j.shots; s0 += NT) {
+    // frame shots: all of the job's shots, or all but the reference shot 0
+    const int64_t f0 = own_ref ? 1 : 0, nfs = j.shots - f0;
+    for (int64_t s0 = 0; s0 < nfs; s0 += NT) {
       ItemDesc it;
       memset(&it, 0, sizeof(it));
       it.seed = j.seed;
@@ -1111,8 +1178,8 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
       it.n_ops = jp.n_ops;
       it.n = n;
       it.d = (uint32_t)j.d;
-      it.S = (uint32_t)std::min<int64_t>(NT, j.shots - s0);
-      it.shot0 = (uint32_t)(shot0 + s0);
+      it.S = (uint32_t)std::min<int64_t>(NT, nfs - s0);
+      it.shot0 = (uint32_t)(shot0 + f0 + s0);
       it.flags = in_lds ? ITEM_LDS : 0;
       it.n_meas = (uint32_t)jp.n_meas;
       it.rec_off = rec_total;
@@ -1133,33 +1200,34 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
   const uint64_t o_ops = align256(o_items + items.size() * sizeof(ItemDesc));
   const uint64_t meta_bytes = std::max<uint64_t>(align256(o_ops + ops.size() * sizeof(DevOp)), 256);
   int rc = ctx_alloc(c, std::max<uint64_t>(off, 256), T->arena);
-  if (rc) return rc;
+  if (rc) return bail(rc);
   rc = ctx_alloc(c, meta_bytes, T->meta);
-  if (rc) { ctx_free(c, T->arena); return rc; }
+  if (rc) { ctx_free(c, T->arena); return bail(rc); }
   uint8_t* meta = (uint8_t*)T->meta.p;
   T->h_meta.assign(meta_bytes, 0);
   if (!items.empty()) memcpy(T->h_meta.data() + o_items, items.data(), items.size() * sizeof(ItemDesc));
   if (!ops.empty()) memcpy(T->h_meta.data() + o_ops, ops.data(), ops.size() * sizeof(DevOp));
-  HIP_OK(hipMemcpyAsync(meta, T->h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
+  FRAME_HIP_OK(hipMemcpyAsync(meta, T->h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
   T->FP.items = (const ItemDesc*)(meta + o_items);
   T->FP.ops = (const DevOp*)(meta + o_ops);
   T->FP.arena = (uint8_t*)T->arena.p;
   T->FP.nt = NT;
   T->frame_lds = lds;
   for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
-  HIP_OK(hipEventCreate(&T->ev0));
-  HIP_OK(hipEventCreate(&T->ev1));
-  HIP_OK(hipEventRecord(T->ev0, T->stream));
+  FRAME_HIP_OK(hipEventCreate(&T->ev0));
+  FRAME_HIP_OK(hipEventCreate(&T->ev1));
+  FRAME_HIP_OK(hipEventRecord(T->ev0, T->stream));
   rc = enqueue(T.get(), 0);
-  if (rc) return rc;
-  HIP_OK(hipEventRecord(T->ev1, T->stream));
+  if (rc) return bail(rc);
+  FRAME_HIP_OK(hipEventRecord(T->ev1, T->stream));
   T->h_rec.resize(T->rec_bytes);
-  if (T->rec_bytes) HIP_OK(hipMemcpyAsync(T->h_rec.data(), T->arena.p, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
+  if (T->rec_bytes) FRAME_HIP_OK(hipMemcpyAsync(T->h_rec.data(), T->arena.p, T->rec_bytes, hipMemcpyDeviceToHost, T->stream));
   T->id = c->next_ticket++;
   *ticket = T->id;
   c->tickets[T->id] = std::move(T);
   return 0;
 }
+#undef FRAME_HIP_OK
 
 int sdim_wait(sdim_ctx* c, uint64_t ticket) {
   if (!c) return fail(SDIM_EINVAL, "null context");
@@ -1175,9 +1243,11 @@ int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) {
   Ticket* t = find_ticket(c, ticket);
   if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
   if (t->finished) { *done = 1; return 0; }
-  const hipError_t e = hipEventQuery(t->ev1);
-  if (e == hipErrorNotReady) { *done = 0; return 0; }
-  if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  for (Ticket* u = t; u; u = u->child.get()) {
+    const hipError_t e = hipEventQuery(u->ev1);
+    if (e == hipErrorNotReady) { *done = 0; return 0; }
+    if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  }
   *done = 1;
   return finish(c, t);
 }
@@ -1213,6 +1283,35 @@ int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int
   const JobPlan& jp = t->jobs[(size_t)job];
   if (capacity < jp.n_meas * jp.shots) return fail(SDIM_EINVAL, "record buffer too small");
   uint8_t* o = (uint8_t*)out;
+  if (t->child) {
+    // shot 0: the reference shot (its tableau ticket); shots 1..: frames, whose records hold
+    // the x component only: M / M_X report (reference value + x) mod d, RESET the reference
+    // value, every shot the reference's deterministic flag (sdim/program.py:124-161)
+    const Ticket* ch = t->child.get();
+    const ItemDesc& ci = ch->items[ch->jobs[(size_t)job].first_item];
+    const int64_t d = jp.d;
+    for (int64_t ev = 0; ev < jp.n_meas; ev++) {
+      const uint8_t rb = ch->h_rec[ci.rec_off + (size_t)ev * ci.S];
+      const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], rv = rb & 0x7F;
+      const uint8_t det = (rb >> 7) & 1;
+      const bool reset = jp.meas_kind[(size_t)ev] == 16;
+      auto put = [&](int64_t shot, int64_t val) {
+        uint8_t* rec = o + 33 * (size_t)(ev * jp.shots + shot);
+        memcpy(rec, &q, 8);
+        memcpy(rec + 8, &round, 8);
+        memcpy(rec + 16, &shot, 8);
+        rec[24] = det;
+        memcpy(rec + 25, &val, 8);
+      };
+      put(0, rv);
+      for (uint32_t i = 0; i < jp.n_items; i++) {
+        const ItemDesc& it = t->items[jp.first_item + i];
+        const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
+        for (uint32_t s = 0; s < it.S; s++) put((int64_t)it.shot0 + s, reset ? rv : (rv + (r[s] & 0x7F)) % d);
+      }
+    }
+    return 0;
+  }
   for (int64_t ev = 0; ev < jp.n_meas; ev++) {
     for (uint32_t i = 0; i < jp.n_items; i++) {
       const ItemDesc& it = t->items[jp.first_item + i];
@@ -1242,8 +1341,12 @@ int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, 
   if (job < 0 || (size_t)job >= t->jobs.size()) return fail(SDIM_ENOTFOUND, "unknown job");
   int rc = finish(c, t);
   if (rc) return rc;
+  if (t->mode == MODE_FRAME) {
+    // only the reference shot of a frame ticket that simulated it has a tableau
+    if (!t->child || shot != 0) return fail(SDIM_EINVAL, "a frame-sampled shot keeps no tableau");
+    t = t->child.get();
+  }
   const JobPlan& jp = t->jobs[(size_t)job];
-  if (t->mode == MODE_FRAME) return fail(SDIM_EINVAL, "a frame-sampled job keeps no tableaux");
   if (shot < 0 || shot >= jp.shots) return fail(SDIM_EINVAL, "shot out of range");
   for (uint32_t i = 0; i < jp.n_items; i++) {
     const ItemDesc& it = t->items[jp.first_item + i];
@@ -1332,7 +1435,13 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
     if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   HIP_OK(hipEventRecord(t->ev0, t->stream));
+  Ticket* ch = t->child.get();
+  if (ch) HIP_OK(hipStreamWaitEvent(ch->stream, t->ev0, 0));
   for (int32_t i = 0; i < iters; i++) {
+    if (ch) {                       // the reference shots, concurrently on their own stream
+      rc = enqueue(ch, 0);
+      if (rc) return rc;
+    }
     if (t->mode == SDIM_MODE_BATCHED) {
       HIP_OK(hipGraphLaunch(t->graph[which], t->stream));
     } else {
@@ -1340,17 +1449,26 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
       if (rc) return rc;
     }
   }
+  if (ch) {
+    HIP_OK(hipEventRecord(ch->ev1, ch->stream));
+    HIP_OK(hipStreamWaitEvent(t->stream, ch->ev1, 0));
+    if (ch->rec_bytes)
+      HIP_OK(hipMemcpyAsync(ch->h_rec.data(), ch->arena.p, ch->rec_bytes, hipMemcpyDeviceToHost, ch->stream));
+  }
   HIP_OK(hipEventRecord(t->ev1, t->stream));
   if (t->d_counters)
     HIP_OK(hipMemcpyAsync(t->h_counters, t->d_counters, sizeof(t->h_counters), hipMemcpyDeviceToHost, t->stream));
   if (t->rec_bytes)
     HIP_OK(hipMemcpyAsync(t->h_rec.data(), t->arena.p, t->rec_bytes, hipMemcpyDeviceToHost, t->stream));
   HIP_OK(hipStreamSynchronize(t->stream));
-  for (size_t gi = 0; gi < t->groups.size(); gi++) {
-    if (!t->classes[t->groups[gi].cls].gang) continue;
-    int e = 0;
-    HIP_OK(hipMemcpy(&e, t->groups[gi].gp.err, sizeof(int), hipMemcpyDeviceToHost));
-    if (e) return fail(SDIM_EHIP, "a gang barrier timed out during replay; results are invalid");
+  if (ch) HIP_OK(hipStreamSynchronize(ch->stream));   // its record copy follows its kernels
+  for (Ticket* u = t; u; u = u->child.get()) {
+    for (size_t gi = 0; gi < u->groups.size(); gi++) {
+      if (!u->classes[u->groups[gi].cls].gang) continue;
+      int e = 0;
+      HIP_OK(hipMemcpy(&e, u->groups[gi].gp.err, sizeof(int), hipMemcpyDeviceToHost));
+      if (e) return fail(SDIM_EHIP, "a gang barrier timed out during replay; results are invalid");
+    }
   }
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));

@@ -181,26 +181,38 @@ class Engine:
     def run(self, jobs, mode="batched"):
         return self.submit(jobs, mode).wait()
 
-    def frame_submit(self, jobs, refs, shot0=1):
-        """Pauli-frame sampling (sdim/program.py:45-165): ``jobs[i].shots`` shots numbered from
-        ``shot0``, each reporting the reference shot's records ``refs[i]`` (MEASUREMENT_DTYPE,
-        one per measurement event in op order) shifted by its frame."""
+    def frame_submit(self, jobs, refs=None, shot0=None):
+        """Pauli-frame sampling (sdim/program.py:45-165, 244-265).
+
+        With ``refs`` (one MEASUREMENT_DTYPE array per job: the reference shot's records, one per
+        measurement event in op order) every one of ``jobs[i].shots`` shots, numbered from
+        ``shot0`` (default 1), is a frame shot reporting those records shifted by its frame.
+        Without ``refs`` the library simulates the reference shot itself (shot 0, N1 as the
+        identity, concurrently with the frames) and ``jobs[i].shots`` counts it: records are
+        (events, shots) with shot 0 the reference shot, whose final tableau ``tableau(i, 0)``
+        returns."""
         jobs = list(jobs)
-        if len(refs) != len(jobs):
-            raise ValueError("one reference record array per job")
         arr = (_lib.SdimJob * max(len(jobs), 1))()
-        ptrs = (ctypes.c_void_p * max(len(jobs), 1))()
-        keep = []
-        for i, (j, r) in enumerate(zip(jobs, refs)):
+        for i, j in enumerate(jobs):
             arr[i] = j.to_c()
-            r = np.ascontiguousarray(np.asarray(r, dtype=MEASUREMENT_DTYPE).reshape(-1))
-            keep.append(r)
-            ptrs[i] = r.ctypes.data if r.size else None
+        keep = []
+        if refs is None:
+            ptrs = None
+            shot0 = 0 if shot0 is None else shot0
+        else:
+            if len(refs) != len(jobs):
+                raise ValueError("one reference record array per job")
+            ptrs = (ctypes.c_void_p * max(len(jobs), 1))()
+            for i, r in enumerate(refs):
+                r = np.ascontiguousarray(np.asarray(r, dtype=MEASUREMENT_DTYPE).reshape(-1))
+                keep.append(r)
+                ptrs[i] = r.ctypes.data if r.size else None
+            shot0 = 1 if shot0 is None else shot0
         tid = ctypes.c_uint64(0)
         _lib.check(self.L.sdim_frame_submit(self.ctx, arr, len(jobs), ptrs, int(shot0), ctypes.byref(tid)), self.L)
         return Ticket(self, tid.value, jobs)
 
-    def frame_run(self, jobs, refs, shot0=1):
+    def frame_run(self, jobs, refs=None, shot0=None):
         return self.frame_submit(jobs, refs, shot0).wait()
 
     def close(self):

@@ -165,17 +165,16 @@ class Program:
         seed = options.seed if options.seed is not None else (self.seed if self.seed is not None
                                                               else random.getrandbits(63))
         self.last_seed = seed
-        ref_job = self._job(ops, 1, seed, self.last_stream, noiseless=True)
-        tk = engine.run([ref_job])
-        ref = tk.records(0)[:, 0]
-        final = tk.tableau(0, 0)
-        tk.release()
         if options.show_gate or options.verbose:
             self._trace(engine, ops, seed, options, noiseless=True)
-        fj = self._job(ops, shots - 1, seed, self.last_stream)
-        tk = engine.frame_run([fj], [ref], shot0=1)
-        rec = tk.records(0)                       # (events, shots - 1)
+        # one submission: the noiseless reference shot (shot 0, on the tableau engine) and the
+        # frame shots 1..shots-1 run concurrently; records come back combined
+        tk = engine.frame_run([self._job(ops, shots, seed, self.last_stream)])
+        rec = tk.records(0)                       # (events, shots), shot 0 = the reference shot
+        final = tk.tableau(0, 0)
         tk.release()
+        ref = rec[:, 0]
+        rec = rec[:, 1:]
 
         self.measurement_results = [[] for _ in range(n)]
         for ev in range(ref.shape[0]):

@@ -147,3 +147,34 @@ def test_repetition_configuration_frames(eng):
     meas = ir[np.isin(ir[:, 0], (14, 15, 16)), 0] == 14
     assert np.array_equal(fr[meas], tab[meas])
     assert (fr['measurement_value'][meas] != 0).any()     # the noise shows
+
+
+@pytest.mark.parametrize("d", [2, 3, 5, 7, 11])
+def test_frame_with_device_reference_shot(eng, d):
+    """Without reference records the library runs the reference shot itself (shot 0, N1 as
+    the identity) concurrently with the frames: shot 0 equals the noiseless tableau oracle,
+    shots 1.. the frame oracle on those records, and tableau(0) the reference final tableau."""
+    rng = np.random.default_rng(2300 + d)
+    jobs = [rand_job(rng, int(n), d, 8 * int(n), shots=int(s), stream=i)
+            for i, (n, s) in enumerate([(5, 1), (9, 300), (60, 70)])]
+    print(f"[frame] device reference d={d}", flush=True)
+    tk = eng.frame_run(jobs)
+    for i, job in enumerate(jobs):
+        ir = np.asarray(job.ir)
+        ir0 = ir.copy()
+        ir0[ir0[:, 0] == 17, 0] = 0
+        rec0, tabs = c_oracle.simulate(job.n, d, ir0, shots=1, seed=job.seed, stream=job.stream, keep_shots=1)
+        got = np.transpose(_stack(tk.records(i)), (1, 0, 2))              # [shots, events, 5]
+        assert np.array_equal(got[0], rec0[0]), f"reference shot differs for job {i}"
+        pairs = [(int(r[4]), bool(r[3])) for r in rec0[0]]
+        exp = c_oracle.frame(job.n, d, ir, pairs, job.shots - 1, job.seed, job.stream, _noise(job), shot0=1)
+        assert np.array_equal(got[1:], exp), f"frame shots differ for job {i}"
+        for a, b in zip(tk.tableau(i, 0), tabs[0]):
+            assert np.array_equal(a, b)
+    before = [tk.records(i).copy() for i in range(len(jobs))]
+    tk.replay(2)
+    for i in range(len(jobs)):
+        assert np.array_equal(tk.records(i), before[i])
+    with pytest.raises(ValueError):
+        tk.tableau(1, 1)
+    tk.release()
