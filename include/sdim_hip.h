@@ -81,11 +81,12 @@ const char* sdim_last_error(void);
  *                     separate tableau per shot)
  *   "grid"            persistent grid size (0 = automatic)
  *   "allow_lds"       0 forces every tableau to stay in HBM
- *   "hbm_threads"     workgroup size of the launch class for small HBM-resident items (64..1024)
+ *   "hbm_threads"     workgroup size of the launch class for small HBM-resident items (64..1024,
+ *                     default 256)
  *   "hbm_per_cu"      LDS budget of that class is 160 KB / hbm_per_cu (default 4); at most
  *                     hbm_per_cu workgroups of it share a CU (fewer when registers limit)
- *   "serve_hbm_per_cu" the same for the submissions of the serving loop (default 2: a larger
- *                     budget per workgroup, lower request latency)
+ *   "serve_hbm_threads", "serve_hbm_per_cu"  the same two for the submissions of the serving
+ *                     loop (defaults 512 and 2: larger workgroups, lower request latency)
  *   "gang_min_n"      circuits with at least this many qudits run on a gang of workgroups,
  *                     each owning a range of generator columns (0 = never; default 1536)
  *   "gang_chunk"      minimum bytes of a row each gang member owns (multiple of 16; default 128,

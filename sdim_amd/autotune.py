@@ -24,17 +24,18 @@ import numpy as np
 
 from .engine import Engine
 
-# candidate values per option, tried one option at a time from the current best
+# candidate values per option (or per tuple of options that interact, tried together), one
+# coordinate at a time from the current best.  Workgroup size and workgroups per CU of the HBM
+# class are searched jointly: registers cap a CU at 16 waves, so the best size depends on the count.
 SPACE = [
-    ("hbm_threads", (512, 256, 1024)),
-    ("hbm_per_cu", (4, 2, 1, 8)),
+    (("hbm_threads", "hbm_per_cu"), ((256, 4), (512, 4), (512, 2), (256, 2), (1024, 1), (128, 8))),
     ("shots_per_item", (0, 1, 2, 4, 8, 16, 32)),
     ("lds_bytes", (163584, 131072, 98304, 65536)),
     ("gang_min_n", (1536, 1024, 2048, 4096, 0)),
     ("gang_chunk", (128, 256, 64)),
     ("gang_threads", (512, 256, 1024)),
 ]
-DEFAULTS = {"hbm_threads": 512, "hbm_per_cu": 4, "shots_per_item": 0, "lds_bytes": 163584,
+DEFAULTS = {"hbm_threads": 256, "hbm_per_cu": 4, "shots_per_item": 0, "lds_bytes": 163584,
             "gang_min_n": 1536, "gang_chunk": 128, "gang_threads": 512}
 # options that only matter for some workloads
 _GANG = ("gang_min_n", "gang_chunk", "gang_threads")
@@ -97,11 +98,13 @@ def tune(jobs, reps=3, space=SPACE, cache=None, verbose=False, max_seconds=120.0
     for key, values in space:
         if key in _GANG and not has_gang:
             continue
+        keys = key if isinstance(key, tuple) else (key,)
         for v in values:
-            if v == best[key] or time.perf_counter() - t_start > max_seconds:
+            vals = v if isinstance(key, tuple) else (v,)
+            if all(best[k] == x for k, x in zip(keys, vals)) or time.perf_counter() - t_start > max_seconds:
                 continue
             cand = dict(best)
-            cand[key] = v
+            cand.update(zip(keys, vals))
             try:
                 ms, _ = _time(cand, jobs, reps, ref)
             except (ValueError, RuntimeError) as e:       # infeasible (e.g. LDS too small) or wrong

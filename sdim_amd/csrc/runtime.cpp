@@ -173,9 +173,11 @@ struct sdim_ctx {
   int64_t shots_per_item = 0;
   int64_t grid = 0;
   int64_t allow_lds = 1;
-  int64_t hbm_threads = 512;
-  int64_t hbm_per_cu = 4;     // LDS budget of the HBM class = 160 KB / 4 (chosen by the autotuner: leaves room for LDS-class workgroups)
-  int64_t serve_hbm_per_cu = 2;  // the same for serving submissions (a larger budget halves p50/p99 latency there)
+  int64_t hbm_threads = 256;
+  int64_t hbm_per_cu = 4;     // LDS budget of the HBM class = 160 KB / 4, 256 threads (chosen by the autotuner:
+                              // four items per CU in flight, room left for LDS-class workgroups)
+  int64_t serve_hbm_threads = 512;  // the same two for serving submissions: fewer, larger workgroups
+  int64_t serve_hbm_per_cu = 2;     // halve p50/p99 request latency there
   int64_t gang_min_n = 1536;  // circuits with at least this many qudits run on gangs (0: never)
   int64_t gang_chunk = 128;   // minimum bytes of each member's share of a row (one cache line)
   int64_t gang_threads = 512;
@@ -695,6 +697,9 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "hbm_per_cu") {
     if (v < 1 || v > 16) return fail(SDIM_EINVAL, "hbm_per_cu must be 1..16");
     c->hbm_per_cu = v;
+  } else if (k == "serve_hbm_threads") {
+    if (v != 64 && v != 128 && v != 256 && v != 512 && v != 1024) return fail(SDIM_EINVAL, "serve_hbm_threads must be 64..1024");
+    c->serve_hbm_threads = v;
   } else if (k == "serve_hbm_per_cu") {
     if (v < 1 || v > 16) return fail(SDIM_EINVAL, "serve_hbm_per_cu must be 1..16");
     c->serve_hbm_per_cu = v;
@@ -731,6 +736,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "hbm_threads") *v = c->hbm_threads;
   else if (k == "hbm_per_cu") *v = c->hbm_per_cu;
   else if (k == "serve_hbm_per_cu") *v = c->serve_hbm_per_cu;
+  else if (k == "serve_hbm_threads") *v = c->serve_hbm_threads;
   else if (k == "num_cu") *v = c->n_cu;
   else if (k == "gang_min_n") *v = c->gang_min_n;
   else if (k == "gang_chunk") *v = c->gang_chunk;
@@ -745,7 +751,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
 
 // Lowers, plans, uploads and enqueues a submission on stream `stream_idx` (caller holds c->mu).
 static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, int stream_idx,
-                         uint64_t* ticket, uint64_t* device_bytes, int64_t hbm_per_cu = 0) {
+                         uint64_t* ticket, uint64_t* device_bytes, int64_t hbm_threads = 0, int64_t hbm_per_cu = 0) {
   if (!c || (!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
   if (mode != SDIM_MODE_BATCHED && mode != SDIM_MODE_PLAIN) return fail(SDIM_EINVAL, "unknown mode");
   HIP_OK(hipSetDevice(c->device));
@@ -754,7 +760,8 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   T->stream = c->streams[(size_t)stream_idx % c->streams.size()];
   T->jobs.resize((size_t)n_jobs);
   const uint32_t cap = (uint32_t)c->lds_bytes;
-  T->classes = make_classes((int)c->hbm_threads, (int)(hbm_per_cu ? hbm_per_cu : c->hbm_per_cu));
+  T->classes = make_classes((int)(hbm_threads ? hbm_threads : c->hbm_threads),
+                            (int)(hbm_per_cu ? hbm_per_cu : c->hbm_per_cu));
   const std::vector<LaunchClass>& kClasses = T->classes;
   const int kNumClasses = (int)kClasses.size();
   uint64_t total_shots = 0;
@@ -1662,7 +1669,8 @@ void serve_loop(sdim_server* s) {
     {
       std::lock_guard<std::mutex> lk(c->mu);
       rc = submit_locked(c, jobs.data(), (int32_t)jobs.size(), SDIM_MODE_BATCHED,
-                         (int)(s->inflight.size() % c->streams.size()), &ticket, &dev_bytes, c->serve_hbm_per_cu);
+                         (int)(s->inflight.size() % c->streams.size()), &ticket, &dev_bytes,
+                         c->serve_hbm_threads, c->serve_hbm_per_cu);
     }
     const double t_launch = now_s();
     std::lock_guard<std::mutex> lk(s->mu);
