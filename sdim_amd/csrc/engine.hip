@@ -1135,9 +1135,11 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 }
 
 // ------------------------------------------------------------------ one item
-template <int D, int NT>
-SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
-  const bool res = (it.flags & ITEM_LDS) != 0;
+// RES: the whole tableau in LDS (a compile-time placement lets its accesses compile to LDS
+// instructions instead of generic ones); otherwise X/Z in HBM and the phases in HBM or LDS.
+template <int D, int NT, bool RES>
+SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
+  constexpr bool res = RES;
   const bool phl = !res && (it.flags & ITEM_PH_LDS) != 0;     // phases kept in LDS
   const uint32_t n = it.n, rs = it.rs, Sh = it.S;
   const uint64_t T_bytes = tableau_bytes(n, D, Sh);
@@ -1198,6 +1200,12 @@ SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
     atomicAdd(&P.counters[res ? 1 : 0], tot);
     if (ltot) atomicAdd(&P.counters[1], ltot);
   }
+}
+
+template <int D, int NT>
+SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
+  if (it.flags & ITEM_LDS) run_item_at<D, NT, true>(it, P, lds);
+  else run_item_at<D, NT, false>(it, P, lds);
 }
 
 // ------------------------------------------------------------------ persistent batch kernel
