@@ -169,7 +169,8 @@ int sdim_ticket_info(sdim_ctx* ctx, uint64_t ticket, const char* key, int64_t* v
 /*
  * Raw device counters of the last run of a ticket: [0] tableau bytes of HBM-resident
  * items, [1] of LDS-resident items, then per-phase clock ticks when the library is
- * built with SDIM_PHASE_TIMERS (zero otherwise).  Returns the number of values written.
+ * built with SDIM_PHASE_TIMERS (zero otherwise; that build is currently switched off).
+ * Returns the number of values written.
  * Diagnostics only; no reference counterpart.
  */
 int sdim_ticket_counters(sdim_ctx* ctx, uint64_t ticket, uint64_t* out, int32_t count);
