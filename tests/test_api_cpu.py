@@ -219,4 +219,10 @@ def test_autotune_signature_and_cache(tmp_path):
             self.set[k] = v
     e = FakeEngine()
     assert autotune.apply(e, jobs, path) == {"hbm_threads": 256} and e.set == {"hbm_threads": 256}
-    assert set(autotune.DEFAULTS) == {k for k, _ in autotune.SPACE}
+    keys = set()
+    for k, values in autotune.SPACE:
+        ks = k if isinstance(k, tuple) else (k,)
+        keys.update(ks)
+        for v in values:           # joint coordinates list one value per option
+            assert len(v if isinstance(k, tuple) else (v,)) == len(ks)
+    assert set(autotune.DEFAULTS) == keys
