@@ -928,14 +928,17 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     SDIM_TADD(P, 2, tC);
     SDIM_TMARK(tD);
 
-    // ---- D: columns p and n+p for every row; per-shot phases over words with f != 0
+    // ---- D: columns p and n+p; per-shot phases over words with f != 0.  Column n+p is
+    // rewritten at every row (the old destabiliser's support is not known), column p only at
+    // the touched rows: elsewhere it is the pivot column's zero already.
+    const bool pg_word = BIN && (p >> 6) == ((n + p) >> 6);   // d = 2: p and n+p in one word
     for (uint32_t r = tid; r < n; r += NT) {
       if constexpr (BIN) {
         const uint64_t xv = S.xp[r], zv = S.zp[r];
         const uint32_t g = n + p;
-        uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
-        uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
-        if ((p >> 6) == (g >> 6)) {
+        if (pg_word) {
+          uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
+          uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
           uint64_t x = *xp_, z = *zp_;
           x &= ~(1ull << (p & 63));
           x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
@@ -945,18 +948,30 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         } else {
           uint64_t* xg_ = w64(T.X, r, g >> 6, rs, T.il);
           uint64_t* zg_ = w64(T.Z, r, g >> 6, rs, T.il);
-          *xp_ &= ~(1ull << (p & 63));
-          *zp_ = (*zp_ & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
           *xg_ = (*xg_ & ~(1ull << (g & 63))) | (xv << (g & 63));
           *zg_ = (*zg_ & ~(1ull << (g & 63))) | (zv << (g & 63));
         }
-        bytes += 32;
+        bytes += 16;
       } else {
-        *b8(T.X, r, p, rs, T.il) = 0;
-        *b8(T.Z, r, p, rs, T.il) = (uint8_t)(r == q);
         *b8(T.X, r, n + p, rs, T.il) = (uint8_t)((e * S.xp[r]) % D);
         *b8(T.Z, r, n + p, rs, T.il) = (uint8_t)((e * S.zp[r]) % D);
-        bytes += 4;
+        bytes += 2;
+      }
+    }
+    if (!pg_word) {
+      for (uint32_t k = tid; k < nT; k += NT) {
+        const uint32_t r = S.TR[k];
+        if constexpr (BIN) {
+          uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
+          uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
+          *xp_ &= ~(1ull << (p & 63));
+          *zp_ = (*zp_ & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
+          bytes += 16;
+        } else {
+          *b8(T.X, r, p, rs, T.il) = 0;
+          *b8(T.Z, r, p, rs, T.il) = (uint8_t)(r == q);
+          bytes += 2;
+        }
       }
     }
     rowsum_phases<D, NT>(T, S, nF, e, xz, pinc, bytes);
