@@ -81,17 +81,23 @@ struct Scr {
   uint8_t* DW;       // segment phase delta: 2 * rs bytes
   uint32_t* red;     // NT * 4 words
   uint32_t* shot;    // 2 * S words
-  uint8_t* noise;    // noise tables of a segment
+  uint8_t* noise;    // noise tables of a segment (over the measurement-only scratch)
   uint8_t* lds_end;  // end of this workgroup's dynamic LDS
+  uint8_t* gap;      // LDS reserved by the caller in front of the measurement-only scratch
   bool tiling;       // stage column tiles of an HBM-resident tableau through LDS for gate segments
 };
 
 template <int NT>
-SDIM_DEV Scr carve(uint8_t* base, uint32_t n, uint32_t rs, uint32_t S) {
+SDIM_DEV Scr carve(uint8_t* base, uint32_t n, uint32_t rs, uint32_t S, uint32_t gap = 0) {
   Scr s;
   uint8_t* p = base;
   s.misc = (int*)p; p += 256;
   s.mtab = (MulTab*)p; p += 16 * 16;
+  s.DW = p; p += 2 * rs;
+  s.red = (uint32_t*)p; p += NT * 16;
+  s.gap = p; p += (gap + 15u) & ~15u;
+  // measurement-only scratch, last: a gate segment overlays it (and whatever LDS follows)
+  // with its noise tables and column tiles
   const uint32_t nb = (n + 15u) & ~15u;
   s.xp = p; p += nb;
   s.zp = p; p += nb;
@@ -100,10 +106,8 @@ SDIM_DEV Scr carve(uint8_t* base, uint32_t n, uint32_t rs, uint32_t S) {
   s.RQ = p; p += rs;
   s.F = p; p += rs;
   s.DOT = (uint32_t*)p; p += 2 * rs;
-  s.DW = p; p += 2 * rs;
-  s.red = (uint32_t*)p; p += NT * 16;
   s.shot = (uint32_t*)p; p += ((2 * S * 4) + 15u) & ~15u;
-  s.noise = p;
+  s.noise = s.xp;
   s.lds_end = nullptr;
   s.tiling = false;
   return s;
@@ -367,8 +371,18 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   // need no coordination); an LDS-resident tableau is a single tile.
   uint32_t TW = rs;
   uint8_t* tile = nullptr;
+  // phases cached in LDS are parked at their global place while the tiles run (a segment
+  // without noise ops touches them only in its final phase pass); the tiles then also
+  // cover their LDS
+  const bool park = S.tiling && (it.flags & ITEM_PH_LDS) && nn == 0;
+  uint8_t* ph_home = P.arena + it.tab_off + 2ull * rows_padded(n) * rs;
+  const uint32_t ph_bytes = (BIN ? 2u : 1u) * Sh * rs;
+  if (park) {
+    copy16<NT>(ph_home, S.gap, ph_bytes);
+    __syncthreads();
+  }
   if (S.tiling) {
-    tile = S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
+    tile = park ? S.gap : S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
     const int64_t left = (int64_t)(S.lds_end - tile);
     const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
     uint64_t tw = room / (2ull * rows_padded(n));
@@ -519,6 +533,11 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   }
   // gate traffic lands in LDS when the tableau was staged through tiles
   if (S.tiling && TW) lbytes += gb; else bytes += gb;
+  if (park) {
+    copy16<NT>(S.gap, ph_home, ph_bytes);
+    __syncthreads();
+    bytes += 2ull * ph_bytes;
+  }
   const uint32_t NWd = T.nw;
   SDIM_TADD(P, 7, tSeg);
   SDIM_TMARK(tPh);
@@ -1161,12 +1180,14 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   const uint64_t XZ_bytes = 2ull * rows_padded(n) * rs, PH_bytes = T_bytes - XZ_bytes;
   uint8_t* gtab = P.arena + it.tab_off;
   uint8_t* tab = res ? lds : gtab;
-  uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : (phl ? lds + ((PH_bytes + 15) & ~15ull) : lds);
-  Scr S = carve<NT>(sbase, n, rs, Sh);
+  // LDS: [whole tableau (res)] counters, tables, segment delta, reductions, [phases (phl)],
+  // measurement-only scratch, free
+  uint8_t* sbase = res ? lds + ((T_bytes + 15) & ~15ull) : lds;
+  Scr S = carve<NT>(sbase, n, rs, Sh, phl ? (uint32_t)PH_bytes : 0u);
   S.lds_end = lds + P.lds_bytes;
   S.tiling = !res && it.il == 3;
   Tab T;
-  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = phl ? lds : tab + XZ_bytes; T.n = n; T.rs = rs; T.S = Sh;
+  T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = phl ? S.gap : tab + XZ_bytes; T.n = n; T.rs = rs; T.S = Sh;
   T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
   unsigned long long bytes = 0, lbytes = 0;
   SDIM_TMARK(tItem);
@@ -1175,7 +1196,7 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   if (threadIdx.x < 32) S.misc[threadIdx.x] = ((threadIdx.x & 15) == 0) ? INT_MAX : 0;
   if (it.flags & ITEM_INIT_GIVEN) {
     if (res) copy16<NT>(lds, gtab, T_bytes);
-    if (phl) copy16<NT>(lds, gtab + XZ_bytes, PH_bytes);
+    if (phl) copy16<NT>(S.gap, gtab + XZ_bytes, PH_bytes);
     __syncthreads();
   } else {
     init_identity<D, NT>(T, XZ_bytes, PH_bytes);
@@ -1204,7 +1225,7 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
     if (threadIdx.x == 0) bytes += T_bytes;
   }
   if (phl) {
-    copy16<NT>(gtab + XZ_bytes, lds, PH_bytes);
+    copy16<NT>(gtab + XZ_bytes, S.gap, PH_bytes);
     if (threadIdx.x == 0) bytes += PH_bytes;
   }
   SDIM_TADD(P, 10, tOut);
