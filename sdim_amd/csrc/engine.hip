@@ -53,7 +53,7 @@ namespace sdim {
 //                     (hiding the draws behind the loads of row q) instead of step E; items in
 //                     LDS always draw in step E, and only when the outcome is random
 //   SDIM_BALLOT_TR    compact the touched-row list with wave ballots instead of per-row LDS atomics
-//   SDIM_DET_BATCH    words per load batch in the deterministic scan
+//   SDIM_DET_BATCH    listed words (f != 0) per load batch in the deterministic scan
 #ifndef SDIM_EARLY_DRAWS
 #define SDIM_EARLY_DRAWS 1
 #endif
@@ -64,7 +64,7 @@ namespace sdim {
 #define SDIM_SWEEP_RB 2   // rows in flight per thread in the one-workgroup sweep
 #endif
 #ifndef SDIM_DET_BATCH
-#define SDIM_DET_BATCH 1
+#define SDIM_DET_BATCH 4
 #endif
 
 // ------------------------------------------------------------------ scratch carve
@@ -1043,29 +1043,53 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         ((uint32_t*)S.F)[w] = f;
       }
     }
-    __syncthreads();
+    // ordered list of the words with f != 0 in FL (f has few nonzero words: the rows and
+    // the shots below visit only those); each thread tests the words it wrote itself
+    uint32_t nfw = 0;
+    {
+      const uint32_t lane = tid & 63, wid = tid >> 6;
+      for (uint32_t w0 = 0; w0 < nsw; w0 += NT) {
+        const uint32_t w = w0 + tid;
+        bool nz = false;
+        if (w < nsw) nz = BIN ? ((const uint64_t*)S.F)[w] != 0ull : ((const uint32_t*)S.F)[w] != 0u;
+        const uint64_t bal = __ballot(nz);
+        if (lane == 0) S.red[wid] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        #pragma unroll
+        for (uint32_t i = 0; i < NT / 64; i++) {
+          const uint32_t c = S.red[i];
+          before += i < wid ? c : 0u;
+          tot += c;
+        }
+        if (nz) S.FL[nfw + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)w;
+        nfw += tot;
+        __syncthreads();
+      }
+    }
     // cross term sum_r sum_i (f x)_ri * prefix_{j<i} (f z)_rj and the exponent term
-    // sum_i f_i (f_i - 1) / 2 <x_i, z_i>, one thread per row
-    // (loads are issued DB words ahead of their use; words with f = 0 are skipped)
+    // sum_i f_i (f_i - 1) / 2 <x_i, z_i>, one thread per row, over the listed words in
+    // order (the loads of DB listed words are issued before their use)
     constexpr uint32_t DB = SDIM_DET_BATCH;
     uint32_t part = 0;
     for (uint32_t r = tid; r < n; r += NT) {
       if constexpr (BIN) {
         uint64_t carry = 0, par = 0;
-        for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
+        for (uint32_t k0 = 0; k0 < nfw; k0 += DB) {
           uint64_t xa[DB], za[DB];
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
-            const uint32_t w = w0 + u;
-            const uint64_t f = w < nsw ? ((const uint64_t*)S.F)[w] : 0ull;
-            xa[u] = f ? *w64(T.X, r, w, rs, T.il) : 0ull;
-            za[u] = f ? *w64(T.Z, r, w, rs, T.il) : 0ull;
+            xa[u] = 0ull; za[u] = 0ull;
+            if (k0 + u < nfw) {
+              const uint32_t w = S.FL[k0 + u];
+              xa[u] = *w64(T.X, r, w, rs, T.il);
+              za[u] = *w64(T.Z, r, w, rs, T.il);
+            }
           }
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
-            const uint32_t w = w0 + u;
-            const uint64_t f = w < nsw ? ((const uint64_t*)S.F)[w] : 0ull;
-            if (!f) continue;
+            if (k0 + u >= nfw) break;
+            const uint64_t f = ((const uint64_t*)S.F)[S.FL[k0 + u]];
             const uint64_t a = xa[u] & f, b = za[u] & f;
             uint64_t y = b;
             y ^= y << 1; y ^= y << 2; y ^= y << 4; y ^= y << 8; y ^= y << 16; y ^= y << 32;
@@ -1078,20 +1102,21 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
         part += (uint32_t)par;
       } else {
         uint32_t U = 0, acc = 0;
-        for (uint32_t w0 = 0; w0 < nsw; w0 += DB) {
+        for (uint32_t k0 = 0; k0 < nfw; k0 += DB) {
           uint32_t xa[DB], za[DB];
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
-            const uint32_t w = w0 + u;
-            const uint32_t f = w < nsw ? ((const uint32_t*)S.F)[w] : 0u;
-            xa[u] = f ? *w32(T.X, r, w, rs, T.il) : 0u;
-            za[u] = f ? *w32(T.Z, r, w, rs, T.il) : 0u;
+            xa[u] = 0u; za[u] = 0u;
+            if (k0 + u < nfw) {
+              const uint32_t w = S.FL[k0 + u];
+              xa[u] = *w32(T.X, r, w, rs, T.il);
+              za[u] = *w32(T.Z, r, w, rs, T.il);
+            }
           }
           #pragma unroll
           for (uint32_t u = 0; u < DB; u++) {
-            const uint32_t w = w0 + u;
-            const uint32_t f = w < nsw ? ((const uint32_t*)S.F)[w] : 0u;
-            if (!f) continue;
+            if (k0 + u >= nfw) break;
+            const uint32_t f = ((const uint32_t*)S.F)[S.FL[k0 + u]];
             #pragma unroll
             for (int b = 0; b < 4; b++) {
               const uint32_t fb = byte_of(f, b);
@@ -1110,9 +1135,9 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       uint32_t m;
       if constexpr (BIN) {
         uint32_t c0 = 0, c1 = 0;
-        for (uint32_t w = 0; w < nsw; w++) {
+        for (uint32_t k = 0; k < nfw; k++) {
+          const uint32_t w = S.FL[k];
           const uint64_t f = ((const uint64_t*)S.F)[w];
-          if (!f) continue;
           c0 += __builtin_popcountll(plane(T, s, 0)[w] & f);
           c1 += __builtin_popcountll(plane(T, s, 1)[w] & f);
           bytes += 16;
@@ -1122,9 +1147,9 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
       } else {
         const uint32_t* ph = (const uint32_t*)phase_vec(T, s);
         uint32_t acc = 0;
-        for (uint32_t w = 0; w < nsw; w++) {
+        for (uint32_t k = 0; k < nfw; k++) {
+          const uint32_t w = S.FL[k];
           const uint32_t f = ((const uint32_t*)S.F)[w];
-          if (!f) continue;
           const uint32_t v = ph[w];
           #pragma unroll
           for (int b = 0; b < 4; b++) acc += byte_of(f, b) * byte_of(v, b);
