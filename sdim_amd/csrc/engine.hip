@@ -536,7 +536,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   if (park) {
     copy16<NT>(S.gap, ph_home, ph_bytes);
     __syncthreads();
-    bytes += 2ull * ph_bytes;
+    if (tid == 0) bytes += 2ull * ph_bytes;
   }
   const uint32_t NWd = T.nw;
   SDIM_TADD(P, 7, tSeg);
