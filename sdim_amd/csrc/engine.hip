@@ -52,13 +52,9 @@ namespace sdim {
 //   SDIM_EARLY_DRAWS  for items whose tableau is in HBM, draw random outcomes during step A
 //                     (hiding the draws behind the loads of row q) instead of step E; items in
 //                     LDS always draw in step E, and only when the outcome is random
-//   SDIM_BALLOT_TR    compact the touched-row list with wave ballots instead of per-row LDS atomics
 //   SDIM_DET_BATCH    listed words (f != 0) per load batch in the deterministic scan
 #ifndef SDIM_EARLY_DRAWS
 #define SDIM_EARLY_DRAWS 1
-#endif
-#ifndef SDIM_BALLOT_TR
-#define SDIM_BALLOT_TR 0
 #endif
 #ifndef SDIM_SWEEP_RB
 #define SDIM_SWEEP_RB 2   // rows in flight per thread in the one-workgroup sweep
@@ -847,65 +843,67 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 
   if (piv != INT_MAX) {
     const uint32_t p = (uint32_t)piv;
-    // ---- B: gather the pivot column, touched rows (compacted per wave), <x_p, z_p>, pivot phases
-    if constexpr (!SDIM_BALLOT_TR) {
-      for (uint32_t r = tid; r < n; r += NT) {
-        uint32_t xr, zr;
-        if constexpr (BIN) {
-          xr = (uint32_t)((*w64(T.X, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
-          zr = (uint32_t)((*w64(T.Z, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
+    // exponentiation of stabilizer p by e = x[q,p]^-1 (tableau_prime.py:278-282, 365-380),
+    // folded into every later use of column p
+    uint32_t e = 1;
+    if constexpr (!BIN) {
+      const uint32_t xv = S.RQ[p];
+      #pragma unroll
+      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
+    }
+    // ---- B: gather the pivot column, touched rows, <x_p, z_p>, pivot phases.  Columns p and
+    // n+p get their final values here, from the values just read: stabilizer p becomes Z_q at
+    // the touched rows (it is zero elsewhere), destabilizer n+p becomes e times stabilizer p at
+    // every row (its old support is not known).  The sweep reads these groups after the full
+    // barrier below and keeps both columns, f being 0 there.
+    const uint32_t np_ = n + p;
+    for (uint32_t r = tid; r < n; r += NT) {
+      uint32_t xr, zr;
+      if constexpr (BIN) {
+        const uint32_t wp = p >> 6, wn = np_ >> 6;
+        uint64_t* xpw = w64(T.X, r, wp, rs, T.il);
+        uint64_t* zpw = w64(T.Z, r, wp, rs, T.il);
+        uint64_t* xnw = w64(T.X, r, wn, rs, T.il);
+        uint64_t* znw = w64(T.Z, r, wn, rs, T.il);
+        const bool sep = wn != wp;
+        const uint64_t xa = *xpw, za = *zpw;
+        const uint64_t xb = sep ? *xnw : xa, zb = sep ? *znw : za;
+        xr = (uint32_t)((xa >> (p & 63)) & 1);
+        zr = (uint32_t)((za >> (p & 63)) & 1);
+        const uint64_t mp = 1ull << (p & 63), mn = 1ull << (np_ & 63);
+        uint64_t xn = (xb & ~mn) | (xr ? mn : 0ull), zn = (zb & ~mn) | (zr ? mn : 0ull);
+        bytes += sep ? 32 : 16;
+        if (!sep && (xr | zr)) {
+          xn &= ~mp;
+          zn = (zn & ~mp) | (r == q ? mp : 0ull);
+        }
+        if (xn != xb) { *xnw = xn; bytes += 8; }
+        if (zn != zb) { *znw = zn; bytes += 8; }
+        if (sep && (xr | zr)) {
+          *xpw = xa & ~mp;
+          *zpw = (za & ~mp) | (r == q ? mp : 0ull);
           bytes += 16;
-        } else {
-          xr = *b8(T.X, r, p, rs, T.il);
-          zr = *b8(T.Z, r, p, rs, T.il);
+        }
+      } else {
+        xr = *b8(T.X, r, p, rs, T.il);
+        zr = *b8(T.Z, r, p, rs, T.il);
+        const uint32_t xo = *b8(T.X, r, np_, rs, T.il), zo = *b8(T.Z, r, np_, rs, T.il);
+        const uint32_t xn = (e * xr) % D, zn = (e * zr) % D;
+        if (xn != xo) { *b8(T.X, r, np_, rs, T.il) = (uint8_t)xn; bytes += 1; }
+        if (zn != zo) { *b8(T.Z, r, np_, rs, T.il) = (uint8_t)zn; bytes += 1; }
+        bytes += 4;
+        if (xr | zr) {
+          *b8(T.X, r, p, rs, T.il) = 0;
+          *b8(T.Z, r, p, rs, T.il) = (uint8_t)(r == q);
           bytes += 2;
         }
-        S.xp[r] = (uint8_t)xr;
-        S.zp[r] = (uint8_t)zr;
-        if (xr | zr) {
-          S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
-          if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
-        }
       }
-    } else {
-      constexpr int GB = 4;
-      const uint32_t lane = tid & 63;
-      uint32_t xzs = 0;
-      for (uint32_t r0 = 0; r0 < n; r0 += NT * GB) {
-        uint32_t xs[GB], zs[GB];
-        #pragma unroll
-        for (int b = 0; b < GB; b++) {
-          const uint32_t r = r0 + (uint32_t)b * NT + tid;
-          xs[b] = 0; zs[b] = 0;
-          if (r < n) {
-            if constexpr (BIN) {
-              xs[b] = (uint32_t)((*w64(T.X, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
-              zs[b] = (uint32_t)((*w64(T.Z, r, p >> 6, rs, T.il) >> (p & 63)) & 1);
-              bytes += 16;
-            } else {
-              xs[b] = *b8(T.X, r, p, rs, T.il);
-              zs[b] = *b8(T.Z, r, p, rs, T.il);
-              bytes += 2;
-            }
-          }
-        }
-        #pragma unroll
-        for (int b = 0; b < GB; b++) {
-          const uint32_t r = r0 + (uint32_t)b * NT + tid;
-          const bool hit = (xs[b] | zs[b]) != 0;
-          const uint64_t bal = __ballot(hit);
-          uint32_t base = 0;
-          if (lane == 0 && bal) base = (uint32_t)atomicAdd(&M[1], (int)__popcll(bal));
-          base = __shfl(base, 0, 64) + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
-          if (r < n) {
-            S.xp[r] = (uint8_t)xs[b];
-            S.zp[r] = (uint8_t)zs[b];
-            if (hit) S.TR[base] = (uint16_t)r;
-            xzs += xs[b] * zs[b];
-          }
-        }
+      S.xp[r] = (uint8_t)xr;
+      S.zp[r] = (uint8_t)zr;
+      if (xr | zr) {
+        S.TR[atomicAdd(&M[1], 1)] = (uint16_t)r;
+        if constexpr (!BIN) atomicAdd(&M[3], (int)(xr * zr));
       }
-      if constexpr (!BIN) if (xzs) atomicAdd(&M[3], (int)(xzs % D));
     }
     if (tid == 0) {
       if constexpr (BIN) {
@@ -929,13 +927,8 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     SDIM_TADD(P, 1, tB);
     SDIM_TMARK(tC);
     const uint32_t nT = (uint32_t)M[1], nF = (uint32_t)M[2];
-    // exponentiation of stabilizer p by e = x[q,p]^-1 (tableau_prime.py:278-282, 365-380),
-    // folded into every later use of column p
-    uint32_t e = 1, xz = 0, pinc = 0;
+    uint32_t xz = 0, pinc = 0;
     if constexpr (!BIN) {
-      const uint32_t xv = S.xp[q];
-      #pragma unroll
-      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
       const uint32_t xz_old = (uint32_t)M[3] % D;
       xz = ((e * e) % D) * xz_old % D;
       pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
@@ -947,52 +940,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     SDIM_TADD(P, 2, tC);
     SDIM_TMARK(tD);
 
-    // ---- D: columns p and n+p; per-shot phases over words with f != 0.  Column n+p is
-    // rewritten at every row (the old destabiliser's support is not known), column p only at
-    // the touched rows: elsewhere it is the pivot column's zero already.
-    const bool pg_word = BIN && (p >> 6) == ((n + p) >> 6);   // d = 2: p and n+p in one word
-    for (uint32_t r = tid; r < n; r += NT) {
-      if constexpr (BIN) {
-        const uint64_t xv = S.xp[r], zv = S.zp[r];
-        const uint32_t g = n + p;
-        if (pg_word) {
-          uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
-          uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
-          uint64_t x = *xp_, z = *zp_;
-          x &= ~(1ull << (p & 63));
-          x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
-          z = (z & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
-          *xp_ = x; *zp_ = z;
-        } else {
-          uint64_t* xg_ = w64(T.X, r, g >> 6, rs, T.il);
-          uint64_t* zg_ = w64(T.Z, r, g >> 6, rs, T.il);
-          *xg_ = (*xg_ & ~(1ull << (g & 63))) | (xv << (g & 63));
-          *zg_ = (*zg_ & ~(1ull << (g & 63))) | (zv << (g & 63));
-        }
-        bytes += 16;
-      } else {
-        *b8(T.X, r, n + p, rs, T.il) = (uint8_t)((e * S.xp[r]) % D);
-        *b8(T.Z, r, n + p, rs, T.il) = (uint8_t)((e * S.zp[r]) % D);
-        bytes += 2;
-      }
-    }
-    if (!pg_word) {
-      for (uint32_t k = tid; k < nT; k += NT) {
-        const uint32_t r = S.TR[k];
-        if constexpr (BIN) {
-          uint64_t* xp_ = w64(T.X, r, p >> 6, rs, T.il);
-          uint64_t* zp_ = w64(T.Z, r, p >> 6, rs, T.il);
-          *xp_ &= ~(1ull << (p & 63));
-          *zp_ = (*zp_ & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          bytes += 16;
-        } else {
-          *b8(T.X, r, p, rs, T.il) = 0;
-          *b8(T.Z, r, p, rs, T.il) = (uint8_t)(r == q);
-          bytes += 2;
-        }
-      }
-    }
+    // ---- D: per-shot phases over words with f != 0
     rowsum_phases<D, NT>(T, S, nF, e, xz, pinc, bytes);
     __syncthreads();                                                      // B4
     SDIM_TADD(P, 3, tD);
