@@ -59,6 +59,9 @@ namespace sdim {
 #ifndef SDIM_SWEEP_RB
 #define SDIM_SWEEP_RB 2   // rows in flight per thread in the one-workgroup sweep
 #endif
+#ifndef SDIM_SEG_OPB
+#define SDIM_SEG_OPB 2   // gate batches of a layer whose ops are loaded in one round trip
+#endif
 #ifndef SDIM_DET_BATCH
 #define SDIM_DET_BATCH 4
 #endif
@@ -334,6 +337,7 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
                               unsigned long long& lbytes) {
   constexpr bool BIN = (D == 2);
   constexpr int GB = 4;     // gates per load batch
+  constexpr int OB = SDIM_SEG_OPB;   // batches whose ops are loaded together
   SDIM_TMARK(tSeg);
   const DevOp hdr = ops[k];
   const uint32_t nl = hdr.q, nn = hdr.t;
@@ -430,15 +434,26 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
     if (active) {
       for (uint32_t w = my_w; w < NWd; w += NT) {
         if (many) { dl = 0; b0 = 0; b1 = 0; }
-        for (uint32_t g0 = my_slot; g0 < L; g0 += NS * GB) {
-          DevOp ob[GB];
+        // the ops of OB consecutive batches are loaded first (one global round trip), then the
+        // rows of each batch of GB gates (LDS or global) before computing it
+        for (uint32_t g0 = my_slot; g0 < L; g0 += NS * GB * OB) {
+          DevOp oa[GB * OB];
+          #pragma unroll
+          for (int b = 0; b < GB * OB; b++) {
+            const uint32_t g = g0 + (uint32_t)b * NS;
+            if (g < L) { const DevOp& src = lops[g]; oa[b].w0 = src.w0; oa[b].q = src.q; oa[b].t = src.t; oa[b].src = 0; }
+          }
+          #pragma unroll
+          for (int h = 0; h < OB; h++) {
+          const uint32_t gh = g0 + (uint32_t)(h * GB) * NS;
+          if (gh >= L) break;
+          const DevOp* ob = oa + h * GB;
           if constexpr (BIN) {
             uint64_t xq[GB], zq[GB], xt[GB], zt[GB];
             #pragma unroll
             for (int b = 0; b < GB; b++) {
-              const uint32_t g = g0 + (uint32_t)b * NS;
+              const uint32_t g = gh + (uint32_t)b * NS;
               if (g < L) {
-                ob[b] = lops[g];
                 xq[b] = *w64(V.X, ob[b].q, w, V.rs, V.il);
                 zq[b] = *w64(V.Z, ob[b].q, w, V.rs, V.il);
                 if (two_rows(ob[b])) { xt[b] = *w64(V.X, ob[b].t, w, V.rs, V.il); zt[b] = *w64(V.Z, ob[b].t, w, V.rs, V.il); }
@@ -447,14 +462,13 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
             }
             #pragma unroll
             for (int b = 0; b < GB; b++)
-              if (g0 + (uint32_t)b * NS < L) gb += apply_bin(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, b0, b1, S);
+              if (gh + (uint32_t)b * NS < L) gb += apply_bin(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, b0, b1, S);
           } else {
             uint32_t xq[GB], zq[GB], xt[GB], zt[GB];
             #pragma unroll
             for (int b = 0; b < GB; b++) {
-              const uint32_t g = g0 + (uint32_t)b * NS;
+              const uint32_t g = gh + (uint32_t)b * NS;
               if (g < L) {
-                ob[b] = lops[g];
                 xq[b] = *w32(V.X, ob[b].q, w, V.rs, V.il);
                 zq[b] = *w32(V.Z, ob[b].q, w, V.rs, V.il);
                 if (two_rows(ob[b])) { xt[b] = *w32(V.X, ob[b].t, w, V.rs, V.il); zt[b] = *w32(V.Z, ob[b].t, w, V.rs, V.il); }
@@ -463,7 +477,8 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
             }
             #pragma unroll
             for (int b = 0; b < GB; b++)
-              if (g0 + (uint32_t)b * NS < L) gb += apply_odd<D>(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, dl, S);
+              if (gh + (uint32_t)b * NS < L) gb += apply_odd<D>(ob[b], xq[b], zq[b], xt[b], zt[b], V, w, dl, S);
+          }
           }
         }
         if (many) {
