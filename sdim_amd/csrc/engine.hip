@@ -57,7 +57,7 @@ namespace sdim {
 #define SDIM_EARLY_DRAWS 1
 #endif
 #ifndef SDIM_SWEEP_RB
-#define SDIM_SWEEP_RB 2   // rows in flight per thread in the one-workgroup sweep
+#define SDIM_SWEEP_RB 1   // rows in flight per thread in the one-workgroup sweep
 #endif
 #ifndef SDIM_SEG_OPB
 #define SDIM_SEG_OPB 2   // gate batches of a layer whose ops are loaded in one round trip
