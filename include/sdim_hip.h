@@ -95,6 +95,10 @@ const char* sdim_last_error(void);
  *   "gang_max"        most workgroups per gang (0 = one per CU)
  *   "frame_lds_bytes" frame sampler: LDS for the frames of one workgroup's 256 shots (one
  *                     byte per qudit and shot; default 65536); larger frames live in device memory
+ *   "gang_timeout"    ticks of the 100 MHz constant clock a gang member waits at a barrier before
+ *                     the gang gives up and the ticket fails (default 1e9 = 10 s)
+ *   "list_cap"        diagnostics: caps the kernels' LDS lists of touched rows and factor groups
+ *                     (0 = their full size); an overflow drops the write and fails the ticket
  */
 int sdim_set_option(sdim_ctx* ctx, const char* key, int64_t value);
 int sdim_get_option(sdim_ctx* ctx, const char* key, int64_t* value);
@@ -185,6 +189,13 @@ int sdim_ticket_counters(sdim_ctx* ctx, uint64_t ticket, uint64_t* out, int32_t 
  * the hot path without host-side lowering and copies.
  */
 int sdim_replay(sdim_ctx* ctx, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms);
+
+/*
+ * Overwrite every measurement record of a finished ticket (device and host copies) with
+ * the byte `value` and clear its counters, so that a following sdim_replay must
+ * recompute them.  Diagnostics only; no reference counterpart.
+ */
+int sdim_ticket_poison(sdim_ctx* ctx, uint64_t ticket, int32_t value);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching of independent requests (one circuit each, starting from

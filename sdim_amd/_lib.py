@@ -14,7 +14,7 @@ EXPORTS = (
     "sdim_ctx_create", "sdim_ctx_destroy", "sdim_last_error", "sdim_set_option", "sdim_get_option",
     "sdim_submit", "sdim_frame_submit", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
     "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats", "sdim_replay",
-    "sdim_ticket_counters", "sdim_ticket_info", "sdim_server_create", "sdim_server_submit", "sdim_server_poll",
+    "sdim_ticket_counters", "sdim_ticket_info", "sdim_ticket_poison", "sdim_server_create", "sdim_server_submit", "sdim_server_poll",
     "sdim_server_wait", "sdim_server_result", "sdim_server_stats", "sdim_server_destroy",
 )
 
@@ -73,6 +73,7 @@ def load(path=None):
     L.sdim_replay.argtypes = [vp, u64, i32, i32, ctypes.POINTER(ctypes.c_double)]
     L.sdim_ticket_counters.argtypes = [vp, u64, ctypes.POINTER(u64), i32]
     L.sdim_ticket_info.argtypes = [vp, u64, ctypes.c_char_p, ctypes.POINTER(i64)]
+    L.sdim_ticket_poison.argtypes = [vp, u64, i32]
     dp = ctypes.POINTER(ctypes.c_double)
     L.sdim_server_create.argtypes = [vp, u64, i32, i32, ctypes.POINTER(vp)]
     L.sdim_server_submit.argtypes = [vp, ctypes.POINTER(SdimJob), ctypes.POINTER(u64)]

@@ -1,4 +1,5 @@
-// Interface between the device engine (engine.hip) and the host runtime.
+// Interface between the device engine (tableau_dev.h and the batch, gang, plain and
+// frame kernels) and the host runtime.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -8,6 +9,15 @@ namespace sdim {
 
 // number of optional phase timers after the two byte counters
 #define SDIM_NUM_TIMERS 16
+// device counters of a launch: [0] tableau bytes of HBM-resident items, [1] of LDS-resident
+// items, [2, 2 + SDIM_NUM_TIMERS) phase timers, then the error word: the first capacity
+// violation of the launch as code << 32 | item id (0: none), which fails the ticket
+#define SDIM_ERR_SLOT (2 + SDIM_NUM_TIMERS)
+#define SDIM_NUM_COUNTERS (SDIM_ERR_SLOT + 1)
+enum KernelError : uint32_t {
+  KERR_TOUCHED_ROWS = 1,    // more touched rows than the touched-row list holds
+  KERR_FACTOR_GROUPS = 2,   // more 16-byte groups with f != 0 than the group list holds
+};
 // launch classes, and item queues (one per launch)
 #define SDIM_MAX_CLASSES 32
 #define SDIM_MAX_QUEUES 64
@@ -20,12 +30,13 @@ struct KParams {
   const DevOp* ops;
   const uint64_t* keep53;
   uint8_t* arena;
-  unsigned long long* counters;   // [0] bytes on HBM-resident tableaux, [1] bytes on LDS-resident tableaux
+  unsigned long long* counters;   // SDIM_NUM_COUNTERS device counters (below)
   uint32_t lds_bytes;
+  uint32_t list_cap;              // nonzero: caps the LDS row / group lists (exercises the overflow check)
 };
 
 // LDS scratch of one workgroup for an item with n qudits, row stride rs, S shots
-// (must match carve() in engine.hip)
+// (must match carve() in tableau_dev.h)
 __host__ __device__ inline uint32_t scratch_bytes(uint32_t n, uint32_t rs, uint32_t S, uint32_t NT) {
   uint32_t b = 256 + 16 * 16;                 // counters, multiplication tables
   b += 2 * ((n + 15u) & ~15u);                // pivot column X, Z
@@ -48,7 +59,7 @@ __host__ __device__ inline uint32_t noise_bytes(uint32_t nn, uint32_t rs, uint32
 
 // ---- gangs: one item processed by K workgroups, each owning a range of column words
 struct GangSync;
-// bytes of one gang's barrier record (GangSync in engine.hip)
+// bytes of one gang's barrier record (GangSync in gang.hip)
 constexpr uint64_t GANG_SYNC_BYTES = 4096;
 struct GangParams {
   GangSync* sync;       // one GANG_SYNC_BYTES record per gang
@@ -58,7 +69,10 @@ struct GangParams {
   uint32_t K;           // members per gang
   uint32_t n_max, S_max;
   int* err;             // set when a gang barrier timed out
+  uint64_t timeout;     // ticks of the 100 MHz constant clock a member waits at a barrier before giving up
 };
+// default barrier wait bound of a gang member: 10 s
+constexpr uint64_t GANG_TIMEOUT_TICKS = 1000000000ull;
 
 __host__ __device__ inline uint64_t gang_align(uint64_t x) { return (x + 255u) & ~255ull; }
 // row stride of the per-member byte tables (16-byte multiples)
@@ -96,5 +110,21 @@ int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st);
 int launch_plain_op(const KParams& P, const DevOp& o, uint32_t n, uint32_t d, uint32_t n_items, uint32_t S,
                     hipStream_t st);
+
+// One object per workgroup size (batch.hip) and per dimension (gang.hip) defines these.
+template <int NT> int prepare_batch();
+template <int NT> int launch_batch_nt(const KParams& P, int grid, hipStream_t st);
+template <int D> int prepare_gang();
+template <int D> int launch_gang_d(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st);
+#define SDIM_EXTERN_BATCH(NT)                 \
+  extern template int prepare_batch<NT>(); \
+  extern template int launch_batch_nt<NT>(const KParams& P, int grid, hipStream_t st);
+SDIM_EXTERN_BATCH(64) SDIM_EXTERN_BATCH(128) SDIM_EXTERN_BATCH(256) SDIM_EXTERN_BATCH(512) SDIM_EXTERN_BATCH(1024)
+#undef SDIM_EXTERN_BATCH
+#define SDIM_EXTERN_GANG(D)                  \
+  extern template int prepare_gang<D>(); \
+  extern template int launch_gang_d<D>(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st);
+SDIM_EXTERN_GANG(2) SDIM_EXTERN_GANG(3) SDIM_EXTERN_GANG(5) SDIM_EXTERN_GANG(7) SDIM_EXTERN_GANG(11)
+#undef SDIM_EXTERN_GANG
 
 }  // namespace sdim

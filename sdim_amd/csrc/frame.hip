@@ -129,6 +129,8 @@ SDIM_DEV void frame_walk(const FrameParams& P, const ItemDesc& it, uint8_t* F, u
   }
 }
 
+}  // namespace
+
 template <int NT>
 __global__ void __launch_bounds__(NT) frame_kernel(FrameParams P) {
   extern __shared__ uint8_t lds[];
@@ -140,8 +142,6 @@ __global__ void __launch_bounds__(NT) frame_kernel(FrameParams P) {
     frame_walk(P, it, P.arena + it.tab_off, tid, NT);
   }
 }
-
-}  // namespace
 
 int launch_frame(const FrameParams& P, uint32_t n_items, uint32_t lds_bytes, hipStream_t st) {
   if (!n_items) return 0;

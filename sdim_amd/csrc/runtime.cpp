@@ -134,8 +134,10 @@ struct Ticket {
   std::vector<int> h_gang_err;                      // one error word per gang group
   std::vector<std::vector<uint8_t>> h_inits;
   unsigned long long* d_counters = nullptr;
-  unsigned long long h_counters[2 + SDIM_NUM_TIMERS] = {0};
+  unsigned long long h_counters[SDIM_NUM_COUNTERS] = {0};
   bool finished = false;
+  int err = 0;                                      // sticky: nonzero once the ticket's results are invalid
+  std::string err_msg;
   int64_t n_lds_items = 0;
   // launch plan (re-enqueued by sdim_replay)
   KParams P{};
@@ -183,6 +185,8 @@ struct sdim_ctx {
   int64_t gang_threads = 512;
   int64_t gang_max = 0;       // most members per gang (0: every CU)
   int64_t frame_lds_bytes = 65536;  // frame sampler: LDS for the frames of one workgroup (else device memory)
+  int64_t gang_timeout = (int64_t)GANG_TIMEOUT_TICKS;  // barrier wait bound of a gang member (100 MHz ticks)
+  int64_t list_cap = 0;       // nonzero: caps the kernels' LDS row / group lists (tests the overflow check)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
   std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
 };
@@ -490,17 +494,44 @@ Ticket* find_ticket(sdim_ctx* c, uint64_t id) {
   return it == c->tickets.end() ? nullptr : it->second.get();
 }
 
+// Marks a ticket's results invalid; every later finish() of it reports the same error.
+int poison(Ticket* t, int code, const std::string& msg) {
+  t->finished = true;
+  t->err = code;
+  t->err_msg = msg;
+  return fail(code, msg);
+}
+
+// The error word of the ticket's last run (a list in LDS would have overflowed: the kernel
+// dropped the write, so the results are invalid).
+int kernel_error(Ticket* t) {
+  const unsigned long long e = t->d_counters ? t->h_counters[SDIM_ERR_SLOT] : 0ull;
+  if (!e) return 0;
+  const uint32_t code = (uint32_t)(e >> 32), item = (uint32_t)e;
+  const char* what = code == KERR_TOUCHED_ROWS ? "touched-row list" : code == KERR_FACTOR_GROUPS ? "factor-group list"
+                                                                                                  : "unknown list";
+  return poison(t, SDIM_EHIP, std::string("kernel capacity check failed: ") + what + " overflow in item " +
+                                  std::to_string(item) + "; results are invalid");
+}
+
+// Waits for a ticket's device work.  The first failure (a HIP error on its stream, a gang
+// barrier that timed out, a failed child ticket) is kept in the ticket and returned by this
+// and every later call, so no fetch hands out the records of a failed run.
 int finish(sdim_ctx* c, Ticket* t) {
-  if (t->finished) return 0;
+  if (t->finished) return t->err ? fail(t->err, t->err_msg) : 0;
   if (t->child) {
     const int rc = finish(c, t->child.get());
-    if (rc) return rc;
+    if (rc) {
+      (void)hipStreamSynchronize(t->stream);
+      return poison(t, rc, g_err);
+    }
   }
-  HIP_OK(hipStreamSynchronize(t->stream));
+  const hipError_t e = hipStreamSynchronize(t->stream);
+  if (e != hipSuccess) return poison(t, SDIM_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   t->finished = true;
-  for (int e : t->h_gang_err)
-    if (e) return fail(SDIM_EHIP, "a gang barrier timed out (a member was not resident); results are invalid");
-  return 0;
+  for (int ge : t->h_gang_err)
+    if (ge) return poison(t, SDIM_EHIP, "a gang barrier timed out (a member was not resident); results are invalid");
+  return kernel_error(t);
 }
 
 // frees a ticket's (and its child's) device memory, graphs and events
@@ -718,6 +749,12 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "frame_lds_bytes") {
     if (v < 0 || v > 163840) return fail(SDIM_EINVAL, "frame_lds_bytes must be in [0, 163840]");
     c->frame_lds_bytes = v;
+  } else if (k == "gang_timeout") {
+    if (v < 1) return fail(SDIM_EINVAL, "gang_timeout must be >= 1 tick");
+    c->gang_timeout = v;
+  } else if (k == "list_cap") {
+    if (v < 0 || v > 65535) return fail(SDIM_EINVAL, "list_cap must be in [0, 65535]");
+    c->list_cap = v;
   } else {
     return fail(SDIM_EINVAL, "unknown option " + k);
   }
@@ -743,6 +780,8 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "gang_threads") *v = c->gang_threads;
   else if (k == "gang_max") *v = c->gang_max;
   else if (k == "frame_lds_bytes") *v = c->frame_lds_bytes;
+  else if (k == "gang_timeout") *v = c->gang_timeout;
+  else if (k == "list_cap") *v = c->list_cap;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -908,6 +947,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0) | (ph_lds ? ITEM_PH_LDS : 0);
       it.n_meas = (uint32_t)jp.n_meas;
       it.il = il;
+      it.id = (uint32_t)items.size();
       it.rec_off = rec_total;
       rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
       if (given) inits.emplace_back((uint32_t)items.size(), init_buf);
@@ -990,6 +1030,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     g.gp.n_max = n_max;
     g.gp.S_max = S_max;
     g.gp.scr_stride = gang_scratch_bytes(L.gang, n_max, S_max);
+    g.gp.timeout = (uint64_t)c->gang_timeout;
     // layout inside the ticket's gang buffer: [error word][sync records][scratch]
     g.gp.err = (int*)(uintptr_t)gang_bytes;
     gang_bytes += 256;
@@ -1005,7 +1046,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   const uint64_t o_queue = align256(o_keep + keep.size() * 8);
   if (T->groups.size() > SDIM_MAX_QUEUES) return fail(SDIM_EINVAL, "too many launch groups");
   const uint64_t o_cnt = align256(o_queue + 4 * SDIM_MAX_QUEUES);
-  const uint64_t meta_bytes = align256(o_cnt + 8 * (2 + SDIM_NUM_TIMERS));
+  const uint64_t meta_bytes = align256(o_cnt + 8 * SDIM_NUM_COUNTERS);
   int rc = ctx_alloc(c, arena_bytes, T->arena);
   if (rc) return rc;
   rc = ctx_alloc(c, meta_bytes, T->meta);
@@ -1050,6 +1091,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   P.arena = arena;
   P.counters = T->d_counters;
   P.lds_bytes = cap;
+  P.list_cap = (uint32_t)c->list_cap;
   T->d_queue = P.queue;
   for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
   T->n_cu = c->n_cu;
@@ -1189,6 +1231,7 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
       it.shot0 = (uint32_t)(shot0 + f0 + s0);
       it.flags = in_lds ? ITEM_LDS : 0;
       it.n_meas = (uint32_t)jp.n_meas;
+      it.id = (uint32_t)items.size();
       it.rec_off = rec_total;
       rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
       items.push_back(it);
@@ -1249,11 +1292,14 @@ int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) {
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
   if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
-  if (t->finished) { *done = 1; return 0; }
+  if (t->finished) { *done = 1; return t->err ? fail(t->err, t->err_msg) : 0; }
   for (Ticket* u = t; u; u = u->child.get()) {
     const hipError_t e = hipEventQuery(u->ev1);
     if (e == hipErrorNotReady) { *done = 0; return 0; }
-    if (e != hipSuccess) return fail(SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+      *done = 1;
+      return poison(t, SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    }
   }
   *done = 1;
   return finish(c, t);
@@ -1420,6 +1466,23 @@ int sdim_ticket_counters(sdim_ctx* c, uint64_t ticket, uint64_t* out, int32_t co
   return m;
 }
 
+int sdim_ticket_poison(sdim_ctx* c, uint64_t ticket, int32_t value) {
+  if (!c || value < 0 || value > 255) return fail(SDIM_EINVAL, "bad arguments");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(c->device));
+  for (Ticket* u = t; u; u = u->child.get()) {
+    if (!u->rec_bytes) continue;
+    std::fill(u->h_rec.begin(), u->h_rec.end(), (uint8_t)value);
+    HIP_OK(hipMemcpy(u->arena.p, u->h_rec.data(), u->rec_bytes, hipMemcpyHostToDevice));
+    std::fill(std::begin(u->h_counters), std::end(u->h_counters), 0ull);
+  }
+  return 0;
+}
+
 int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms) {
   if (!c || iters < 1 || which < 0 || which > 2) return fail(SDIM_EINVAL, "bad replay arguments");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -1474,8 +1537,12 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
       if (!u->classes[u->groups[gi].cls].gang) continue;
       int e = 0;
       HIP_OK(hipMemcpy(&e, u->groups[gi].gp.err, sizeof(int), hipMemcpyDeviceToHost));
-      if (e) return fail(SDIM_EHIP, "a gang barrier timed out during replay; results are invalid");
+      if (e) return poison(t, SDIM_EHIP, "a gang barrier timed out during replay; results are invalid");
     }
+  }
+  for (Ticket* u = t; u; u = u->child.get()) {
+    rc = kernel_error(u);
+    if (rc) return u == t ? rc : poison(t, rc, g_err);
   }
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
@@ -1571,16 +1638,43 @@ void serve_loop(sdim_server* s) {
     std::vector<Batch> still;
     for (auto& b : s->inflight) {
       bool done = false;
+      int rc = 0;
+      std::string err;
       {
         std::lock_guard<std::mutex> lk(c->mu);
         Ticket* t = find_ticket(c, b.ticket);
-        if (t && hipEventQuery(t->ev1) == hipSuccess) {
-          (void)finish(c, t);
+        const hipError_t q = t ? hipEventQuery(t->ev1) : hipErrorInvalidValue;
+        if (q != hipErrorNotReady) {
+          // finished, or failed (an event error other than not-ready ends the batch too)
+          rc = !t ? fail(SDIM_ENOTFOUND, "serving ticket vanished")
+                  : q != hipSuccess ? poison(t, SDIM_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(q))
+                                    : finish(c, t);
+          if (rc) err = g_err;
           done = true;
         }
       }
       if (!done) { still.push_back(b); continue; }
       const double t_done = now_s();
+      if (rc) {
+        // the whole batch failed: its requests report the error instead of records
+        {
+          std::lock_guard<std::mutex> lk(c->mu);
+          auto it = c->tickets.find(b.ticket);
+          if (it != c->tickets.end()) drop_ticket(c, it);
+        }
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->used -= std::min(s->used, b.bytes);
+          for (uint64_t id : b.reqs) {
+            Request& r = s->reqs[id];
+            r.state = 3;
+            r.error = err;
+            r.t_done = t_done;
+          }
+        }
+        s->cv.notify_all();
+        continue;
+      }
       std::vector<std::pair<uint64_t, std::vector<uint8_t>>> out;
       {
         std::lock_guard<std::mutex> lk(c->mu);

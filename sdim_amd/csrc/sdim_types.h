@@ -88,6 +88,7 @@ struct ItemDesc {
   uint32_t flags;
   uint32_t n_meas;
   uint32_t il;         // X/Z row interleave: 3 (8-row blocks) or 0 (row-major)
+  uint32_t id;         // index of the item in its submission (reported with kernel errors)
 };
 
 static inline __host__ __device__ uint32_t row_stride(uint32_t n, uint32_t d) {

@@ -73,18 +73,21 @@ class Ticket:
         self.jobs = jobs
         self._released = False
 
+    def _check(self, rc):
+        _lib.check(rc, self.engine.L)
+
     def wait(self):
-        _lib.check(self.engine.L.sdim_wait(self.engine.ctx, self.id))
+        self._check(self.engine.L.sdim_wait(self.engine.ctx, self.id))
         return self
 
     def done(self):
         flag = ctypes.c_int32(0)
-        _lib.check(self.engine.L.sdim_poll(self.engine.ctx, self.id, ctypes.byref(flag)))
+        self._check(self.engine.L.sdim_poll(self.engine.ctx, self.id, ctypes.byref(flag)))
         return bool(flag.value)
 
     def num_measurements(self, job=0):
         c = ctypes.c_int64(0)
-        _lib.check(self.engine.L.sdim_num_measurements(self.engine.ctx, self.id, job, ctypes.byref(c)))
+        self._check(self.engine.L.sdim_num_measurements(self.engine.ctx, self.id, job, ctypes.byref(c)))
         return c.value
 
     def records(self, job=0):
@@ -93,7 +96,7 @@ class Ticket:
         shots = int(self.jobs[job].shots)
         out = np.zeros((m, shots), dtype=MEASUREMENT_DTYPE)
         if m:
-            _lib.check(self.engine.L.sdim_fetch_records(self.engine.ctx, self.id, job, out.ctypes.data, m * shots))
+            self._check(self.engine.L.sdim_fetch_records(self.engine.ctx, self.id, job, out.ctypes.data, m * shots))
         return out
 
     def tableau(self, job=0, shot=0):
@@ -101,7 +104,7 @@ class Ticket:
         n = int(self.jobs[job].n)
         arrs = [np.zeros((n, n), np.int64), np.zeros((n, n), np.int64), np.zeros(n, np.int64),
                 np.zeros((n, n), np.int64), np.zeros((n, n), np.int64), np.zeros(n, np.int64)]
-        _lib.check(self.engine.L.sdim_fetch_tableau(self.engine.ctx, self.id, job, shot,
+        self._check(self.engine.L.sdim_fetch_tableau(self.engine.ctx, self.id, job, shot,
                                                     *[a.ctypes.data for a in arrs]))
         return tuple(arrs)
 
@@ -111,7 +114,7 @@ class Ticket:
         bl = ctypes.c_uint64(0)
         ni = ctypes.c_int64(0)
         nl = ctypes.c_int64(0)
-        _lib.check(self.engine.L.sdim_ticket_stats(self.engine.ctx, self.id, ctypes.byref(ms), ctypes.byref(bh),
+        self._check(self.engine.L.sdim_ticket_stats(self.engine.ctx, self.id, ctypes.byref(ms), ctypes.byref(bh),
                                                    ctypes.byref(bl), ctypes.byref(ni), ctypes.byref(nl)))
         return {"kernel_ms": ms.value, "bytes_hbm": bh.value, "bytes_lds": bl.value,
                 "items": ni.value, "lds_items": nl.value}
@@ -119,7 +122,7 @@ class Ticket:
     def info(self, key):
         """Device layout of the ticket: "items", "lds_items", "gang_items", "gang_members", "gangs", "launches"."""
         v = ctypes.c_int64(0)
-        _lib.check(self.engine.L.sdim_ticket_info(self.engine.ctx, self.id, key.encode(), ctypes.byref(v)))
+        self._check(self.engine.L.sdim_ticket_info(self.engine.ctx, self.id, key.encode(), ctypes.byref(v)))
         return v.value
 
     def counters(self):
@@ -127,20 +130,25 @@ class Ticket:
         buf = (ctypes.c_uint64 * 18)()
         m = self.engine.L.sdim_ticket_counters(self.engine.ctx, self.id, buf, 18)
         if m < 0:
-            _lib.check(m)
+            self._check(m)
         return [int(buf[i]) for i in range(m)]
 
     def replay(self, iters=1, which="all"):
         """Re-run the device work `iters` times; returns device milliseconds for all iterations."""
         w = {"all": 0, "hbm": 1, "lds": 2}[which]
         ms = ctypes.c_double(0)
-        _lib.check(self.engine.L.sdim_replay(self.engine.ctx, self.id, int(iters), w, ctypes.byref(ms)))
+        self._check(self.engine.L.sdim_replay(self.engine.ctx, self.id, int(iters), w, ctypes.byref(ms)))
         return ms.value
+
+    def poison_records(self, value=0xFF):
+        """Overwrite the ticket's records (device and host copies) with `value` and clear its
+        counters, so that only a replay that does the work restores them (diagnostics)."""
+        self._check(self.engine.L.sdim_ticket_poison(self.engine.ctx, self.id, int(value)))
 
     def release(self):
         if not self._released:
             self._released = True
-            _lib.check(self.engine.L.sdim_release(self.engine.ctx, self.id))
+            self._check(self.engine.L.sdim_release(self.engine.ctx, self.id))
 
     def __del__(self):
         try:

@@ -78,3 +78,25 @@ def clifford_job(rng, n, d, depth, shots, stream, meas_rounds=1, noise_p=0.0):
             ops.append((14 if rng.random() < 0.7 else 16, q, -1)); ch.append(0); pr.append(0.0)
     return Job(n, d, np.asarray(ops, np.int64), shots=shots, seed=int(rng.integers(1 << 62)), stream=stream,
                noise_channel=np.asarray(ch, np.int32), noise_prob=np.asarray(pr, np.float64))
+
+
+def records_array(rec):
+    """MEASUREMENT_DTYPE records (events, shots) -> int64 [shots, events, 5] (the oracle's layout)."""
+    got = np.stack([rec['qudit_index'], rec['meas_round'], rec['shot'],
+                    rec['deterministic'].astype(np.int64), rec['measurement_value']], axis=-1)
+    return np.ascontiguousarray(np.transpose(got, (1, 0, 2)), dtype=np.int64)
+
+
+def digest_records(arr):
+    """16-byte SHA-256 prefix of int64 records [shots, events, 5]."""
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(arr, dtype=np.int64).tobytes()).digest()[:16]
+
+
+def digest_tableau(tab):
+    """16-byte SHA-256 prefix of a tableau (x, z, ph, dx, dz, dph) as int64 in the reference layout."""
+    import hashlib
+    h = hashlib.sha256()
+    for a in tab:
+        h.update(np.ascontiguousarray(a, dtype=np.int64).tobytes())
+    return h.digest()[:16]

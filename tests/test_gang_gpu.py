@@ -160,3 +160,41 @@ def test_large_tableau_config_gang_sizes_agree():
     assert np.array_equal(out[0], out[1])
     # X/Z blocks are shared by the shots, so whether an outcome is random does not depend on the shot
     assert np.all(out[0][:, :, 3] == out[0][:, :1, 3])
+
+
+def test_gang_timeout_fails_ticket_and_server_requests():
+    """A gang whose barrier wait bound is forced to one tick gives up at its first barrier
+    (every member sees the error word and leaves the kernel normally).  The ticket then fails
+    for good (wait, a repeated fetch, a replay), and a serving request that ran on such a gang
+    reports failure (state 3) instead of records."""
+    from sdim_amd._lib import EngineError
+    from sdim_amd.engine import Engine
+    from sdim_amd.serve import Server
+    rng = np.random.default_rng(77)
+    jobs = [clifford_job(rng, 200, 3, 600, shots=2, stream=0, meas_rounds=2)]
+    eng = Engine(gang_min_n=48, gang_chunk=16, gang_timeout=1)
+    print("[gang] forced barrier timeout", flush=True)
+    tk = eng.submit(jobs)
+    for _ in range(2):
+        with pytest.raises(EngineError, match="gang barrier timed out"):
+            tk.wait()
+    with pytest.raises(EngineError, match="gang barrier timed out"):
+        tk.records(0)
+    with pytest.raises(EngineError, match="gang barrier timed out"):
+        tk.replay(1)
+    with pytest.raises(EngineError):
+        tk.release()
+    srv = Server(eng, hbm_budget_bytes=1 << 34)
+    rid = srv.submit(jobs[0])
+    with pytest.raises((ValueError, EngineError), match="gang barrier timed out"):
+        srv.wait(rid, timeout=120.0)
+    assert srv.state(rid) == 3
+    srv.close()
+    eng.close()
+    # the same circuit with the default bound is exact
+    eng = Engine(gang_min_n=48, gang_chunk=16)
+    tk = eng.run(jobs)
+    assert tk.info("gang_items") == 1
+    check_ticket(tk, jobs)
+    tk.release()
+    eng.close()

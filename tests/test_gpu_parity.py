@@ -60,19 +60,85 @@ def test_spilling_tableaux(d):
     tk.release()
 
 
-def test_replay_reproduces_records():
+def test_replay_recomputes_records():
+    """Replays recompute every record: the record arena is poisoned before each replay, and
+    afterwards the records match the oracle again and the byte counters are nonzero."""
     from sdim_amd.engine import Engine
     eng = Engine()
     rng = np.random.default_rng(5)
     jobs = [clifford_job(rng, n, d, 3 * n, shots=4, stream=i) for i, (n, d) in enumerate([(30, 2), (60, 3), (220, 5)])]
     tk = eng.run(jobs)
-    before = [tk.records(i).copy() for i in range(len(jobs))]
     for which in ("all", "hbm", "lds", "all"):
+        tk.poison_records(0xFF)
+        assert all((tk.records(i)['measurement_value'] == 127).all() for i in range(len(jobs)))
         tk.replay(2, which)
-        for i in range(len(jobs)):
-            assert np.array_equal(tk.records(i), before[i])
+        st = tk.stats()
+        if which == "all":
+            assert st["bytes_hbm"] > 0 and st["bytes_lds"] > 0
+            check_ticket(tk, jobs, tableaux=False)
+        else:
+            assert (st["bytes_hbm"] if which == "hbm" else st["bytes_lds"]) > 0
     check_ticket(tk, jobs)
     tk.release()
+
+
+def test_uneven_batch_all_circuits_digests():
+    """Every circuit of the uneven batch (4096 circuits, n ~ U[32,512], d ~ U{2,3,5,7}, 10n ops
+    + measure-all, 8 shots) in one submission, then one graph replay: each circuit's records of
+    all 8 shots and shot 0's final tableau against the oracle's digests
+    (tests/golden/make_uneven_digests.py)."""
+    import os
+    from parity_helpers import digest_records, digest_tableau, records_array
+    from sdim_amd import workloads as W
+    from sdim_amd.engine import Engine
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "uneven_batch_digests.npz"))
+    wl = W.uneven_batch()
+    assert [j.n for j in wl.jobs] == list(g["n"]) and [j.d for j in wl.jobs] == list(g["d"])
+    eng = Engine()
+    print("[parity-uneven-all] submitting 4096 circuits", flush=True)
+    tk = eng.run(wl.jobs)
+    for rnd in range(2):
+        bad = []
+        for i, j in enumerate(wl.jobs):
+            if digest_records(records_array(tk.records(i))) != g["records"][i].tobytes():
+                bad.append((i, "records"))
+            if rnd == 0 and digest_tableau(tk.tableau(i, 0)) != g["tableau0"][i].tobytes():
+                bad.append((i, "tableau"))
+        print(f"[parity-uneven-all] round {rnd}: {len(bad)} mismatches", flush=True)
+        assert not bad, f"{len(bad)} circuits differ from the oracle, first {bad[:8]}"
+        if rnd == 0:
+            tk.poison_records(0xFF)
+            tk.replay(1)
+    tk.release()
+    eng.close()
+
+
+def test_list_overflow_fails_ticket_for_good():
+    """A kernel list that would overflow (forced from the host with a tiny list_cap) drops the
+    write, every kernel exits normally, and the ticket fails: wait, a repeated fetch and a
+    replay all report the error."""
+    from sdim_amd._lib import EngineError
+    from sdim_amd.engine import Engine
+    eng = Engine(list_cap=1)
+    rng = np.random.default_rng(17)
+    jobs = [clifford_job(rng, 40, 3, 200, shots=2, stream=0)]
+    tk = eng.submit(jobs)
+    for _ in range(2):
+        with pytest.raises(EngineError, match="capacity check failed"):
+            tk.wait()
+    with pytest.raises(EngineError, match="capacity check failed"):
+        tk.records(0)
+    with pytest.raises(EngineError, match="capacity check failed"):
+        tk.replay(1)
+    with pytest.raises(EngineError):
+        tk.release()
+    eng.close()
+    # the same circuit without the cap is exact
+    eng = Engine()
+    tk = eng.run(jobs)
+    check_ticket(tk, jobs)
+    tk.release()
+    eng.close()
 
 
 def test_uneven_batch_workload_sample():
