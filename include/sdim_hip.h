@@ -131,6 +131,26 @@ int sdim_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, int32_t mod
  */
 int sdim_frame_submit(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
                       uint64_t* ticket);
+
+/*
+ * Batched simulation that allocates nothing: a variant of sdim_submit (batched mode) for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ *   out        caller-owned device buffer of out_capacity bytes.  Job j's records come first,
+ *              at offset r_j (r_0 = 0, r_{j+1} = r_j + n_meas_j * shots_j rounded up to 256):
+ *              n_meas_j x shots_j bytes, event-major, each value | deterministic << 7; the final
+ *              tableaux follow (in the engine's packed layout).  out == NULL: only *out_bytes is
+ *              set (the bytes the jobs need); no device work.
+ *   out_bytes  optional; receives the bytes the jobs need.
+ * The launch metadata lives in a per-GPU device array of 8 MiB (SDIM_NOALLOC_SCRATCH_BYTES)
+ * held by the library; jobs whose metadata exceeds it, that need more than `out_capacity`,
+ * or whose shots do not fit one work item (at most 4096, and the LDS scratch of one item)
+ * are rejected with an error.  Every job runs as one work item on one workgroup (no gangs).
+ * The call waits for the work (on the context's first stream) before it returns.  Calls
+ * that overlap on one GPU (from different contexts) share that array and race.
+ */
+int sdim_simulate_noalloc(sdim_ctx* ctx, const sdim_job* jobs, int32_t n_jobs, void* out, uint64_t out_capacity,
+                          uint64_t* out_bytes);
 int sdim_wait(sdim_ctx* ctx, uint64_t ticket);
 int sdim_poll(sdim_ctx* ctx, uint64_t ticket, int32_t* done);
 int sdim_release(sdim_ctx* ctx, uint64_t ticket);

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libsdim_hip.so")
 # every symbol declared in include/sdim_hip.h
 EXPORTS = (
     "sdim_ctx_create", "sdim_ctx_destroy", "sdim_last_error", "sdim_set_option", "sdim_get_option",
-    "sdim_submit", "sdim_frame_submit", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
+    "sdim_submit", "sdim_frame_submit", "sdim_simulate_noalloc", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
     "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats", "sdim_replay",
     "sdim_ticket_counters", "sdim_ticket_info", "sdim_ticket_poison", "sdim_server_create", "sdim_server_submit", "sdim_server_poll",
     "sdim_server_wait", "sdim_server_result", "sdim_server_stats", "sdim_server_destroy",
@@ -62,6 +62,7 @@ def load(path=None):
     L.sdim_get_option.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i64)]
     L.sdim_submit.argtypes = [vp, ctypes.POINTER(SdimJob), i32, i32, ctypes.POINTER(u64)]
     L.sdim_frame_submit.argtypes = [vp, ctypes.POINTER(SdimJob), i32, ctypes.POINTER(vp), i64, ctypes.POINTER(u64)]
+    L.sdim_simulate_noalloc.argtypes = [vp, ctypes.POINTER(SdimJob), i32, vp, u64, ctypes.POINTER(u64)]
     L.sdim_wait.argtypes = [vp, u64]
     L.sdim_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
     L.sdim_release.argtypes = [vp, u64]

@@ -25,7 +25,8 @@ def units():
     """(object name, source, extra defines) of every translation unit."""
     out = [(f"batch_{nt}", "batch.hip", [f"-DSDIM_BATCH_NT={nt}"]) for nt in BATCH_SIZES]
     out += [(f"gang_{d}", "gang.hip", [f"-DSDIM_GANG_D={d}"]) for d in GANG_DIMS]
-    out += [("plain", "plain.hip", []), ("frame", "frame.hip", []), ("runtime", "runtime.cpp", [])]
+    out += [("plain", "plain.hip", []), ("frame", "frame.hip", []), ("noalloc", "noalloc.hip", []),
+            ("runtime", "runtime.cpp", [])]
     return out
 
 

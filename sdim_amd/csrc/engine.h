@@ -105,6 +105,11 @@ int launch_frame(const FrameParams& P, uint32_t n_items, uint32_t lds_bytes, hip
 int prepare_frame_kernel();
 
 int prepare_kernels();
+
+// allocation-free path (noalloc.hip): device metadata of its submissions, per GPU
+#define SDIM_NOALLOC_SCRATCH_BYTES (8u << 20)
+int noalloc_init_device(int device);     // with `device` current; 0 on success
+uint8_t* noalloc_scratch(int device);    // the device's scratch array (nullptr before init)
 int launch_gang(const KParams& P, const GangParams& G, int d, int nt, int grid, hipStream_t st);
 int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);
 int launch_batch(const KParams& P, int nt, int grid, hipStream_t st);

@@ -676,6 +676,7 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
   }
   HIP_OK(hipEventCreateWithFlags(&c->gang_done, hipEventDisableTiming));
   if (prepare_kernels() || prepare_frame_kernel()) return fail(SDIM_EHIP, "could not set the LDS limit of the kernels");
+  if (noalloc_init_device(device)) return fail(SDIM_EHIP, "could not find the allocation-free path's device scratch");
   *out = c.release();
   return 0;
 }
@@ -788,13 +789,34 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
 
 }  // extern "C"
 
-// Lowers, plans, uploads and enqueues a submission on stream `stream_idx` (caller holds c->mu).
-static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, int stream_idx,
-                         uint64_t* ticket, uint64_t* device_bytes, int64_t hbm_threads = 0, int64_t hbm_per_cu = 0) {
-  if (!c || (!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
+namespace {
+
+// What the planner decides beyond the ticket's items and groups: the host staging of the
+// device metadata (item descriptors, launch order, lowered ops, noise thresholds, then
+// room for the launch queues and counters) and the device layout of a submission.
+struct Staging {
+  std::vector<uint64_t> keep;
+  std::vector<DevOp> ops;
+  std::vector<uint32_t> order;
+  std::vector<std::pair<uint32_t, std::vector<uint8_t>>> inits;   // item -> initial tableau bytes
+  uint64_t arena_bytes = 0;   // records, then tableaux
+  uint64_t gang_bytes = 0;    // gang barrier records and scratch (offsets stored in the groups)
+  uint64_t o_items = 0, o_order = 0, o_ops = 0, o_keep = 0, o_queue = 0, o_cnt = 0, meta_bytes = 0;
+  int used_branches = 0;      // parallel branch streams the batch groups were spread over
+};
+
+struct PlanOpts {
+  bool one_item_per_job = false;   // every job is one item holding all its shots (else split by shots_per_item)
+  bool no_gangs = false;           // never place a circuit on a gang
+  bool branches = true;            // spread batch groups over the stream's parallel branches
+};
+
+// Lowers and places a submission: fills T's jobs, items, classes and groups and the staging
+// (no device memory is touched).  Caller holds c->mu.
+int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, int stream_idx,
+                int64_t hbm_threads, int64_t hbm_per_cu, const PlanOpts& opts, Ticket* T, Staging& st) {
+  if (!c || (!jobs && n_jobs) || n_jobs < 0) return fail(SDIM_EINVAL, "bad submit arguments");
   if (mode != SDIM_MODE_BATCHED && mode != SDIM_MODE_PLAIN) return fail(SDIM_EINVAL, "unknown mode");
-  HIP_OK(hipSetDevice(c->device));
-  std::unique_ptr<Ticket> T(new Ticket());
   T->mode = mode;
   T->stream = c->streams[(size_t)stream_idx % c->streams.size()];
   T->jobs.resize((size_t)n_jobs);
@@ -809,12 +831,12 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   const uint64_t slots = (uint64_t)c->n_cu * 4;
   const uint32_t auto_S = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, total_shots / (2 * slots)));
 
-  std::vector<uint64_t> keep;
-  std::vector<DevOp> ops;
+  std::vector<uint64_t>& keep = st.keep;
+  std::vector<DevOp>& ops = st.ops;
   std::vector<ItemDesc>& items = T->items;
   std::vector<uint64_t> cost;
   std::vector<int> cls_of;
-  std::vector<std::pair<uint32_t, std::vector<uint8_t>>> inits;   // item -> initial tableau bytes
+  auto& inits = st.inits;
 
   uint64_t rec_total = 0;
   for (int32_t ji = 0; ji < n_jobs; ji++) {
@@ -839,9 +861,13 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = row_stride(n, d);
     const bool has_noise = keep.size() > keep0;
     uint32_t S = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : auto_S);
+    if (opts.one_item_per_job) {
+      if (j.shots > 4096) return fail(SDIM_EINVAL, "at most 4096 shots per job on this path (one item per job)");
+      S = (uint32_t)j.shots;
+    }
     int cls = -1;
     uint32_t gang_cb = 0;     // gang items: LDS width of a member's columns
-    if (mode == SDIM_MODE_BATCHED && c->gang_min_n > 0 && j.n >= c->gang_min_n) {
+    if (mode == SDIM_MODE_BATCHED && !opts.no_gangs && c->gang_min_n > 0 && j.n >= c->gang_min_n) {
       // a gang of K workgroups, each owning >= gang_chunk bytes of every row; gang
       // workgroups take the whole LDS of a CU, so one is resident per CU
       const int gnt = (int)c->gang_threads;
@@ -886,6 +912,9 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
       if (cls < 0) {
         // the last class keeps the tableau in HBM with the largest scratch: shrink S until it fits
         const int k = kNumClasses - 1;
+        if (opts.one_item_per_job && need(k, S) > std::min<uint32_t>(kClasses[k].lds, cap))
+          return fail(SDIM_ENOMEM, "job " + std::to_string(ji) + ": its " + std::to_string(S) +
+                                       " shots do not fit one item's LDS scratch on this path");
         while (S > 1 && need(k, S) > std::min<uint32_t>(kClasses[k].lds, cap)) S /= 2;
         if (need(k, S) > std::min<uint32_t>(kClasses[k].lds, cap))
           return fail(SDIM_ENOMEM, "circuit too large for the LDS scratch budget");
@@ -965,9 +994,10 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     it.tab_off = off;
     off = align256(off + tableau_bytes(it.n, it.d, it.S));
   }
-  const uint64_t arena_bytes = std::max<uint64_t>(off, 256);
+  st.arena_bytes = std::max<uint64_t>(off, 256);
   // metadata: items, order, ops, keep, queue, counters.  Order: by launch class, longest first inside a class.
-  std::vector<uint32_t> order(items.size());
+  std::vector<uint32_t>& order = st.order;
+  order.resize(items.size());
   for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     if (cls_of[a] != cls_of[b]) return cls_of[a] < cls_of[b];
@@ -990,7 +1020,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   }
   // batch groups over parallel branches: longest first onto the least loaded branch
   {
-    const int n_br = 1 + (int)c->branch_streams[(size_t)stream_idx % c->branch_streams.size()].size();
+    const int n_br = opts.branches ? 1 + (int)c->branch_streams[(size_t)stream_idx % c->branch_streams.size()].size() : 1;
     std::vector<size_t> gi;
     for (size_t i = 0; i < T->groups.size(); i++) {
       Group& g = T->groups[i];
@@ -1008,14 +1038,10 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     }
     const auto& bs = c->branch_streams[(size_t)stream_idx % c->branch_streams.size()];
     T->branches.assign(bs.begin(), bs.begin() + used_br);
-    for (int k = 0; k <= used_br && used_br > 0; k++) {
-      hipEvent_t e = nullptr;
-      HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      T->fork_join.push_back(e);
-    }
+    st.used_branches = used_br;
   }
   // gang groups: as many gangs as fit on the device at once (and no more than items)
-  uint64_t gang_bytes = 0;
+  uint64_t& gang_bytes = st.gang_bytes;
   for (Group& g : T->groups) {
     const LaunchClass& L = T->classes[g.cls];
     if (!L.gang) continue;
@@ -1039,15 +1065,66 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
     g.gp.scr = (uint8_t*)(uintptr_t)gang_bytes;
     gang_bytes += g.gp.scr_stride * n_gangs;
   }
-  const uint64_t o_items = 0;
-  const uint64_t o_order = align256(o_items + items.size() * sizeof(ItemDesc));
-  const uint64_t o_ops = align256(o_order + order.size() * 4);
-  const uint64_t o_keep = align256(o_ops + ops.size() * sizeof(DevOp));
-  const uint64_t o_queue = align256(o_keep + keep.size() * 8);
+  st.o_items = 0;
+  st.o_order = align256(st.o_items + items.size() * sizeof(ItemDesc));
+  st.o_ops = align256(st.o_order + order.size() * 4);
+  st.o_keep = align256(st.o_ops + ops.size() * sizeof(DevOp));
+  st.o_queue = align256(st.o_keep + keep.size() * 8);
   if (T->groups.size() > SDIM_MAX_QUEUES) return fail(SDIM_EINVAL, "too many launch groups");
-  const uint64_t o_cnt = align256(o_queue + 4 * SDIM_MAX_QUEUES);
-  const uint64_t meta_bytes = align256(o_cnt + 8 * SDIM_NUM_COUNTERS);
-  int rc = ctx_alloc(c, arena_bytes, T->arena);
+  st.o_cnt = align256(st.o_queue + 4 * SDIM_MAX_QUEUES);
+  st.meta_bytes = align256(st.o_cnt + 8 * SDIM_NUM_COUNTERS);
+  for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
+  T->n_cu = c->n_cu;
+  T->grid_override = (int)c->grid;
+  T->threads = (int)c->threads;
+  return 0;
+}
+
+// host staging of the device metadata (queue and counters are cleared by the reset kernel)
+void stage_meta(const Ticket* T, const Staging& st, std::vector<uint8_t>& h_meta) {
+  h_meta.assign(st.meta_bytes, 0);
+  if (!T->items.empty()) memcpy(h_meta.data() + st.o_items, T->items.data(), T->items.size() * sizeof(ItemDesc));
+  if (!st.order.empty()) memcpy(h_meta.data() + st.o_order, st.order.data(), st.order.size() * 4);
+  if (!st.ops.empty()) memcpy(h_meta.data() + st.o_ops, st.ops.data(), st.ops.size() * sizeof(DevOp));
+  if (!st.keep.empty()) memcpy(h_meta.data() + st.o_keep, st.keep.data(), st.keep.size() * 8);
+}
+
+// kernel parameters of a planned submission whose metadata sits at `meta` and arena at `arena`
+void bind_params(sdim_ctx* c, Ticket* T, const Staging& st, uint8_t* meta, uint8_t* arena) {
+  KParams& P = T->P;
+  P.items = (const ItemDesc*)(meta + st.o_items);
+  P.order = (const uint32_t*)(meta + st.o_order);
+  P.n_items = (uint32_t)T->items.size();
+  P.queue = (uint32_t*)(meta + st.o_queue);
+  P.ops = (const DevOp*)(meta + st.o_ops);
+  P.keep53 = (const uint64_t*)(meta + st.o_keep);
+  P.arena = arena;
+  P.counters = (unsigned long long*)(meta + st.o_cnt);
+  P.lds_bytes = (uint32_t)c->lds_bytes;
+  P.list_cap = (uint32_t)c->list_cap;
+  T->d_counters = P.counters;
+  T->d_queue = P.queue;
+}
+
+}  // namespace
+
+// Lowers, plans, uploads and enqueues a submission on stream `stream_idx` (caller holds c->mu).
+static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, int stream_idx,
+                         uint64_t* ticket, uint64_t* device_bytes, int64_t hbm_threads = 0, int64_t hbm_per_cu = 0) {
+  if (!c || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
+  HIP_OK(hipSetDevice(c->device));
+  std::unique_ptr<Ticket> T(new Ticket());
+  Staging st;
+  int rc = plan_locked(c, jobs, n_jobs, mode, stream_idx, hbm_threads, hbm_per_cu, PlanOpts(), T.get(), st);
+  if (rc) return rc;
+  for (int k = 0; k <= st.used_branches && st.used_branches > 0; k++) {
+    hipEvent_t e = nullptr;
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    T->fork_join.push_back(e);
+  }
+  const std::vector<ItemDesc>& items = T->items;
+  const uint64_t arena_bytes = st.arena_bytes, gang_bytes = st.gang_bytes, meta_bytes = st.meta_bytes;
+  rc = ctx_alloc(c, arena_bytes, T->arena);
   if (rc) return rc;
   rc = ctx_alloc(c, meta_bytes, T->meta);
   if (rc) { ctx_free(c, T->arena); return rc; }
@@ -1064,39 +1141,17 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   }
   uint8_t* meta = (uint8_t*)T->meta.p;
   uint8_t* arena = (uint8_t*)T->arena.p;
-  std::vector<uint8_t>& h_meta = T->h_meta;
-  h_meta.assign(meta_bytes, 0);
-  if (!items.empty()) memcpy(h_meta.data() + o_items, items.data(), items.size() * sizeof(ItemDesc));
-  if (!order.empty()) memcpy(h_meta.data() + o_order, order.data(), order.size() * 4);
-  if (!ops.empty()) memcpy(h_meta.data() + o_ops, ops.data(), ops.size() * sizeof(DevOp));
-  if (!keep.empty()) memcpy(h_meta.data() + o_keep, keep.data(), keep.size() * 8);
-  HIP_OK(hipMemcpyAsync(meta, h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
-  for (auto& kv : inits) {
+  stage_meta(T.get(), st, T->h_meta);
+  HIP_OK(hipMemcpyAsync(meta, T->h_meta.data(), meta_bytes, hipMemcpyHostToDevice, T->stream));
+  for (auto& kv : st.inits) {
     const ItemDesc& it = items[kv.first];
     T->h_inits.push_back(std::move(kv.second));
     HIP_OK(hipMemcpyAsync(arena + it.tab_off, T->h_inits.back().data(), T->h_inits.back().size(),
                           hipMemcpyHostToDevice, T->stream));
   }
-  T->d_counters = (unsigned long long*)(meta + o_cnt);
+  bind_params(c, T.get(), st, meta, arena);
   HIP_OK(hipEventCreate(&T->ev0));
   HIP_OK(hipEventCreate(&T->ev1));
-
-  KParams& P = T->P;
-  P.items = (const ItemDesc*)(meta + o_items);
-  P.order = (const uint32_t*)(meta + o_order);
-  P.n_items = (uint32_t)items.size();
-  P.queue = (uint32_t*)(meta + o_queue);
-  P.ops = (const DevOp*)(meta + o_ops);
-  P.keep53 = (const uint64_t*)(meta + o_keep);
-  P.arena = arena;
-  P.counters = T->d_counters;
-  P.lds_bytes = cap;
-  P.list_cap = (uint32_t)c->list_cap;
-  T->d_queue = P.queue;
-  for (auto& it : items) if (it.flags & ITEM_LDS) T->n_lds_items++;
-  T->n_cu = c->n_cu;
-  T->grid_override = (int)c->grid;
-  T->threads = (int)c->threads;
 
   HIP_OK(hipEventRecord(T->ev0, T->stream));
   rc = enqueue(T.get(), 0, c->gang_done);
@@ -1118,12 +1173,64 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   return 0;
 }
 
+// The allocation-free variant of a batched submission (sdim_simulate_noalloc, noalloc.hip):
+// one item per job, no gangs, launches in sequence on the context's first stream; the
+// metadata goes to `scratch` (a device buffer of `scratch_cap` bytes owned by the
+// library) and the arena is the caller's `out`.  Waits for the work before returning.
+int simulate_noalloc_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, void* out, uint64_t out_capacity,
+                            uint64_t* out_bytes, uint8_t* scratch, uint64_t scratch_cap) {
+  if (!c || !scratch) return fail(SDIM_EINVAL, "no scratch buffer for this device");
+  HIP_OK(hipSetDevice(c->device));
+  Ticket T;
+  Staging st;
+  PlanOpts opts;
+  opts.one_item_per_job = true;
+  opts.no_gangs = true;
+  opts.branches = false;
+  int rc = plan_locked(c, jobs, n_jobs, SDIM_MODE_BATCHED, 0, 0, 0, opts, &T, st);
+  if (rc) return rc;
+  if (out_bytes) *out_bytes = st.arena_bytes;
+  if (!out) return 0;   // size query
+  if (out_capacity < st.arena_bytes)
+    return fail(SDIM_EINVAL, "output buffer holds " + std::to_string(out_capacity) + " bytes; the jobs need " +
+                                 std::to_string(st.arena_bytes));
+  if (st.meta_bytes > scratch_cap)
+    return fail(SDIM_ENOMEM, "the jobs' device metadata needs " + std::to_string(st.meta_bytes) +
+                                 " bytes; the allocation-free path has " + std::to_string(scratch_cap));
+  std::vector<uint8_t> h_meta;
+  stage_meta(&T, st, h_meta);
+  HIP_OK(hipMemcpyAsync(scratch, h_meta.data(), st.meta_bytes, hipMemcpyHostToDevice, T.stream));
+  for (auto& kv : st.inits)
+    HIP_OK(hipMemcpyAsync((uint8_t*)out + T.items[kv.first].tab_off, kv.second.data(), kv.second.size(),
+                          hipMemcpyHostToDevice, T.stream));
+  bind_params(c, &T, st, scratch, (uint8_t*)out);
+  rc = enqueue(&T, 0, nullptr);   // the reset kernel clears the queues and counters first
+  if (rc) {
+    (void)hipStreamSynchronize(T.stream);
+    return rc;
+  }
+  HIP_OK(hipStreamSynchronize(T.stream));
+  unsigned long long e = 0;
+  HIP_OK(hipMemcpy(&e, T.P.counters + SDIM_ERR_SLOT, sizeof(e), hipMemcpyDeviceToHost));
+  T.d_counters = T.P.counters;
+  T.h_counters[SDIM_ERR_SLOT] = e;
+  return kernel_error(&T);
+}
+
 extern "C" {
 
 int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) {
   if (!c) return fail(SDIM_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
   return submit_locked(c, jobs, n_jobs, mode, 0, ticket, nullptr);
+}
+
+int sdim_simulate_noalloc(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, void* out, uint64_t out_capacity,
+                          uint64_t* out_bytes) {
+  if (!c) return fail(SDIM_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return simulate_noalloc_locked(c, jobs, n_jobs, out, out_capacity, out_bytes, noalloc_scratch(c->device),
+                                 SDIM_NOALLOC_SCRATCH_BYTES);
 }
 
 int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,

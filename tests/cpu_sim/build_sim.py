@@ -29,6 +29,7 @@ FLAGS = {
 def units():
     u = [(f"batch_{nt}", os.path.join(CSRC, "batch.hip"), [f"-DSDIM_BATCH_NT={nt}"]) for nt in (64, 128, 256, 512, 1024)]
     u += [("plain", os.path.join(CSRC, "plain.hip"), []), ("frame", os.path.join(CSRC, "frame.hip"), []),
+          ("noalloc", os.path.join(CSRC, "noalloc.hip"), []),
           ("runtime", os.path.join(CSRC, "runtime.cpp"), []), ("simt", os.path.join(HERE, "simt.cpp"), []),
           ("gang_stub", os.path.join(HERE, "gang_stub.cpp"), []),
           ("selftest", os.path.join(HERE, "selftest.cpp"), [])]

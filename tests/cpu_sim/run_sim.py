@@ -29,7 +29,7 @@ import numpy as np  # noqa: E402
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--flavor", default="plain")
-    ap.add_argument("--case", default="random", choices=["random", "uneven", "selftest"])
+    ap.add_argument("--case", default="random", choices=["random", "uneven", "selftest", "noalloc"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--count", type=int, default=12)
     ap.add_argument("--hbm", action="store_true", help="random case: also with every tableau in HBM")
@@ -55,6 +55,28 @@ def main():
         print(f"[sim] selftest {a.which}: {L.sdim_sim_selftest(a.which)}", flush=True)
         return
     t0 = time.time()
+    if a.case == "noalloc":
+        import ctypes
+        from noalloc_helpers import check_noalloc_against_submit
+        eng = SimEngine(a.flavor, **opts)
+        rng = np.random.default_rng(a.seed)
+
+        def alloc(nbytes):
+            buf = (ctypes.c_uint8 * max(nbytes, 1))()
+            alloc.keep.append(buf)
+            return ctypes.addressof(buf)
+        alloc.keep = []
+
+        def to_host(ptr, nbytes):
+            return np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(ptr)).copy()
+
+        def fill(ptr, value, nbytes):
+            ctypes.memset(ptr, value, nbytes)
+
+        check_noalloc_against_submit(eng, rng, alloc, to_host, fill)
+        print(f"[sim] noalloc ok ({time.time() - t0:.1f} s)", flush=True)
+        eng.close()
+        return
     if a.case == "random":
         rng = np.random.default_rng(a.seed)
         for d in (2, 3, 5, 7, 11):
