@@ -66,6 +66,8 @@ def parse():
                     help="skip the leg with a separate tableau per shot (shots_per_item=1)")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner on the workload first and measure with its choice")
+    ap.add_argument("--defaults", action="store_true",
+                    help="use the engine's default options instead of the tuned ones shipped for the workload")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="requests/s (0: 60%% of a measured burst rate)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -358,10 +360,18 @@ def main():
     if args.threads:
         opts["threads"] = args.threads
     tuned = None
+    options_source = "defaults"
+    from sdim_amd import autotune
     if args.tune:
-        from sdim_amd import autotune
         tuned = autotune.tune(jobs, reps=max(1, min(args.steps, 3)), max_seconds=150.0)
         opts.update(tuned["options"])
+        options_source = "autotuned in this run"
+    elif not args.defaults and not args.shots_per_item and not args.threads:
+        shipped = autotune.cached(jobs)
+        if shipped:
+            opts.update(shipped)
+            tuned = {"options": shipped, "default_ms": None}
+            options_source = "sdim_amd/tuning.json (autotuner result for this workload)"
     eng = Engine(**opts)
 
     t_submit = time.perf_counter()
@@ -412,6 +422,7 @@ def main():
                    "xz_shared_by_shots": sum(int(j.shots) for j in jobs) / st["items"],
                    "threads": eng.get_option("threads"), "shots_per_item": eng.get_option("shots_per_item"),
                    "options": ({k: eng.get_option(k) for k in tuned["options"]} if tuned else "defaults"),
+                   "options_source": options_source,
                    "tuned_from_ms": tuned["default_ms"] if tuned else None},
         "device_ms_per_step": dev_ms / args.steps,
         "first_submit_s": t_submit,

@@ -112,18 +112,21 @@ int noalloc_init_device(int device);     // with `device` current; 0 on success
 uint8_t* noalloc_scratch(int device);    // the device's scratch array (nullptr before init)
 int launch_gang(const KParams& P, const GangParams& G, int d, int nt, int grid, hipStream_t st);
 int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st);
-int launch_batch(const KParams& P, int nt, int grid, hipStream_t st);
+// every item of a batch launch has the placement `resident` (tableau in LDS) or not; HBM-resident
+// launches keep rb rows in flight per thread in the sweep (1, 2, 4) and load the ops of opb gate
+// batches at once (1, 2)
+int launch_batch(const KParams& P, int nt, bool resident, int rb, int opb, int grid, hipStream_t st);
 int launch_plain_op(const KParams& P, const DevOp& o, uint32_t n, uint32_t d, uint32_t n_items, uint32_t S,
                     hipStream_t st);
 
 // One object per workgroup size (batch.hip) and per dimension (gang.hip) defines these.
 template <int NT> int prepare_batch();
-template <int NT> int launch_batch_nt(const KParams& P, int grid, hipStream_t st);
+template <int NT> int launch_batch_nt(const KParams& P, bool resident, int rb, int opb, int grid, hipStream_t st);
 template <int D> int prepare_gang();
 template <int D> int launch_gang_d(const KParams& P, const GangParams& G, int nt, int grid, hipStream_t st);
 #define SDIM_EXTERN_BATCH(NT)                 \
   extern template int prepare_batch<NT>(); \
-  extern template int launch_batch_nt<NT>(const KParams& P, int grid, hipStream_t st);
+  extern template int launch_batch_nt<NT>(const KParams& P, bool resident, int rb, int opb, int grid, hipStream_t st);
 SDIM_EXTERN_BATCH(64) SDIM_EXTERN_BATCH(128) SDIM_EXTERN_BATCH(256) SDIM_EXTERN_BATCH(512) SDIM_EXTERN_BATCH(1024)
 #undef SDIM_EXTERN_BATCH
 #define SDIM_EXTERN_GANG(D)                  \

@@ -130,13 +130,13 @@ int launch_reset(uint32_t* queue, unsigned long long* counters, hipStream_t st) 
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_batch(const KParams& P, int nt, int grid, hipStream_t st) {
+int launch_batch(const KParams& P, int nt, bool resident, int rb, int opb, int grid, hipStream_t st) {
   switch (nt) {
-    case 64: return launch_batch_nt<64>(P, grid, st);
-    case 128: return launch_batch_nt<128>(P, grid, st);
-    case 256: return launch_batch_nt<256>(P, grid, st);
-    case 512: return launch_batch_nt<512>(P, grid, st);
-    case 1024: return launch_batch_nt<1024>(P, grid, st);
+    case 64: return launch_batch_nt<64>(P, resident, rb, opb, grid, st);
+    case 128: return launch_batch_nt<128>(P, resident, rb, opb, grid, st);
+    case 256: return launch_batch_nt<256>(P, resident, rb, opb, grid, st);
+    case 512: return launch_batch_nt<512>(P, resident, rb, opb, grid, st);
+    case 1024: return launch_batch_nt<1024>(P, resident, rb, opb, grid, st);
   }
   return -2;
 }

@@ -92,18 +92,21 @@ struct LaunchClass {
   int per_cu;      // workgroups resident per CU
   uint32_t gang = 0;  // > 0: items processed by gangs of this many workgroups (gang_kernel)
   int d = 0;          // gang classes: the dimension of their items
+  int rb = 1;         // HBM-resident classes: sweep rows in flight per thread
+  int opb = 2;        //                       gate batches per op load
 };
-// default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu")
-std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu) {
+// default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu"),
+// and the HBM classes' sweep depth and op batching (options "sweep_rows", "seg_batches")
+std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu, int rb, int opb) {
   auto budget = [](int per_cu) { return (uint32_t)((163840 / per_cu - 256) & ~15); };
   return {
       {true, 128, budget(8), 8},
       {true, 256, budget(4), 4},
       {true, 512, budget(2), 2},
       {true, 1024, 163584, 1},
-      {false, hbm_nt, budget(hbm_per_cu), hbm_per_cu},
-      {false, 256, budget(4), 4},
-      {false, 1024, 163584, 1},
+      {false, hbm_nt, budget(hbm_per_cu), hbm_per_cu, 0, 0, rb, opb},
+      {false, 256, budget(4), 4, 0, 0, rb, opb},
+      {false, 1024, 163584, 1, 0, 0, rb, opb},
   };
 }
 
@@ -111,7 +114,7 @@ struct Group {
   int branch = 0;      // which parallel branch launches it (batch groups)
   uint64_t cost = 0;   // estimated work (LPT cost sum)
   int cls = 0;
-  int d = 0;           // gang groups: the dimension of their items (0: mixed)
+  int d = 0;           // gang groups: the dimension of their items
   uint32_t first = 0, count = 0;
   GangParams gp{};     // gang classes: barrier records, scratch, error word
   int gang_grid = 0;   // gang classes: workgroups launched (gangs x members)
@@ -187,6 +190,8 @@ struct sdim_ctx {
   int64_t frame_lds_bytes = 65536;  // frame sampler: LDS for the frames of one workgroup (else device memory)
   int64_t gang_timeout = (int64_t)GANG_TIMEOUT_TICKS;  // barrier wait bound of a gang member (100 MHz ticks)
   int64_t list_cap = 0;       // nonzero: caps the kernels' LDS row / group lists (tests the overflow check)
+  int64_t sweep_rows = 1;     // HBM-resident items: rows in flight per thread in the rowsum sweep (1, 2, 4)
+  int64_t seg_batches = 2;    // HBM-resident items: gate batches whose ops are loaded together (1, 2)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
   std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
 };
@@ -613,7 +618,7 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
         if (hipStreamWaitEvent(st, T->fork_join[0], 0) != hipSuccess) return fail(SDIM_EHIP, "branch fork failed");
         used[b] = true;
       }
-      if (launch_batch(Q, nt, grid, st)) return fail(SDIM_EHIP, "batch kernel launch failed");
+      if (launch_batch(Q, nt, L.resident, L.rb, L.opb, grid, st)) return fail(SDIM_EHIP, "batch kernel launch failed");
     }
     }
     return 0;
@@ -753,6 +758,12 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "gang_timeout") {
     if (v < 1) return fail(SDIM_EINVAL, "gang_timeout must be >= 1 tick");
     c->gang_timeout = v;
+  } else if (k == "sweep_rows") {
+    if (v != 1 && v != 2 && v != 4) return fail(SDIM_EINVAL, "sweep_rows must be 1, 2 or 4");
+    c->sweep_rows = v;
+  } else if (k == "seg_batches") {
+    if (v != 1 && v != 2) return fail(SDIM_EINVAL, "seg_batches must be 1 or 2");
+    c->seg_batches = v;
   } else if (k == "list_cap") {
     if (v < 0 || v > 65535) return fail(SDIM_EINVAL, "list_cap must be in [0, 65535]");
     c->list_cap = v;
@@ -783,6 +794,8 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "frame_lds_bytes") *v = c->frame_lds_bytes;
   else if (k == "gang_timeout") *v = c->gang_timeout;
   else if (k == "list_cap") *v = c->list_cap;
+  else if (k == "sweep_rows") *v = c->sweep_rows;
+  else if (k == "seg_batches") *v = c->seg_batches;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -822,7 +835,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   T->jobs.resize((size_t)n_jobs);
   const uint32_t cap = (uint32_t)c->lds_bytes;
   T->classes = make_classes((int)(hbm_threads ? hbm_threads : c->hbm_threads),
-                            (int)(hbm_per_cu ? hbm_per_cu : c->hbm_per_cu));
+                            (int)(hbm_per_cu ? hbm_per_cu : c->hbm_per_cu), (int)c->sweep_rows, (int)c->seg_batches);
   const std::vector<LaunchClass>& kClasses = T->classes;
   const int kNumClasses = (int)kClasses.size();
   uint64_t total_shots = 0;
@@ -999,6 +1012,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   std::vector<uint32_t>& order = st.order;
   order.resize(items.size());
   for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+  // one group (kernel launch) per launch class, longest items first
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     if (cls_of[a] != cls_of[b]) return cls_of[a] < cls_of[b];
     return cost[a] > cost[b];

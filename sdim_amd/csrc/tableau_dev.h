@@ -57,12 +57,9 @@ namespace sdim {
 #ifndef SDIM_EARLY_DRAWS
 #define SDIM_EARLY_DRAWS 1
 #endif
-#ifndef SDIM_SWEEP_RB
-#define SDIM_SWEEP_RB 1   // rows in flight per thread in the one-workgroup sweep
-#endif
-#ifndef SDIM_SEG_OPB
-#define SDIM_SEG_OPB 2   // gate batches of a layer whose ops are loaded in one round trip
-#endif
+// Per-launch variants (template parameters, chosen per launch class at run time):
+//   RB   rows of the rowsum sweep each thread keeps in flight (16 bytes of X and Z per row)
+//   OPB  gate batches of a layer whose ops are loaded in one round trip
 #ifndef SDIM_DET_BATCH
 #define SDIM_DET_BATCH 4
 #endif
@@ -346,13 +343,13 @@ SDIM_DEV bool two_rows(const DevOp& o) {
 
 // ------------------------------------------------------------------ gate segment
 // ops[k] is the OP_SEG header.  Returns the index after the segment.
-template <int D, int NT>
+template <int D, int NT, int OPB = 2>
 SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const Scr& S,
                               const ItemDesc& it, const KParams& P, unsigned long long& bytes,
                               unsigned long long& lbytes) {
   constexpr bool BIN = (D == 2);
   constexpr int GB = 4;     // gates per load batch
-  constexpr int OB = SDIM_SEG_OPB;   // batches whose ops are loaded together
+  constexpr int OB = OPB;   // batches whose ops are loaded together
   SDIM_TMARK(tSeg);
   const DevOp hdr = ops[k];
   const uint32_t nl = hdr.q, nn = hdr.t;
@@ -846,12 +843,11 @@ SDIM_DEV void rowsum_phases(const Tab& V, const Scr& S, uint32_t nF, uint32_t e,
 // Z-basis measurement of qudit q (tableau_prime.py:262-363) followed, for
 // RESET, by X^((-m) mod d) (program.py:334-339).  Writes one record per shot.
 // mk counts the measurements of the item (selects the counter set).
-template <int D, int NT>
+template <int D, int NT, int RB = 1>
 SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, bool reset, uint32_t mk,
                       const Scr& S, const ItemDesc& it, const KParams& P, unsigned long long& bytes) {
   constexpr bool BIN = (D == 2);
   constexpr uint32_t CPW = BIN ? 64 : 4;
-  constexpr int RB = SDIM_SWEEP_RB;   // rows per load batch in the sweep (16 bytes of X and Z each)
   const uint32_t tid = threadIdx.x;
   const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t G = 2 * n;
@@ -1180,7 +1176,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
 // ------------------------------------------------------------------ one item
 // RES: the whole tableau in LDS (a compile-time placement lets its accesses compile to LDS
 // instructions instead of generic ones); otherwise X/Z in HBM and the phases in HBM or LDS.
-template <int D, int NT, bool RES>
+template <int D, int NT, bool RES, int RB = 1, int OPB = 2>
 SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   constexpr bool res = RES;
   const bool phl = !res && (it.flags & ITEM_PH_LDS) != 0;     // phases kept in LDS
@@ -1222,9 +1218,9 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
     const DevOp o = ops[k];
     const uint32_t kind = op_kind(o);
     if (kind == OP_SEG) {
-      k = run_segment<D, NT>(ops, k, T, S, it, P, bytes, lbytes);
+      k = run_segment<D, NT, OPB>(ops, k, T, S, it, P, bytes, lbytes);
     } else if (kind == OP_MEAS || kind == OP_RESET) {
-      measure<D, NT>(T, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, bytes);
+      measure<D, NT, RB>(T, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, bytes);
       k++;
     } else {
       break;   // OP_END
@@ -1248,12 +1244,6 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
     atomicAdd(&P.counters[res ? 1 : 0], tot);
     if (ltot) atomicAdd(&P.counters[1], ltot);
   }
-}
-
-template <int D, int NT>
-SDIM_DEV void run_item(const ItemDesc& it, const KParams& P, uint8_t* lds) {
-  if (it.flags & ITEM_LDS) run_item_at<D, NT, true>(it, P, lds);
-  else run_item_at<D, NT, false>(it, P, lds);
 }
 
 }  // namespace sdim
