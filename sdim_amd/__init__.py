@@ -13,7 +13,7 @@ Every shot runs on the GPU through the C ABI in include/sdim_hip.h
 (libsdim_hip.so, built with ``python -m sdim_amd.build``).
 """
 from .circuit import Circuit, CircuitInstruction
-from .circuit_io import read_circuit, write_circuit
+from .circuit_io import CircuitIR, read_circuit, read_ir, read_job, write_circuit
 from .engine import MEASUREMENT_DTYPE, Engine, Job, Ticket, default_engine
 from .gatedata import Gate, GateData
 from .program import Program, SimulationOptions
@@ -21,7 +21,8 @@ from .random_circuit import generate_and_write_random_circuit, generate_random_c
 from .tableau import ExtendedTableau, MeasurementResult, Tableau
 
 __all__ = [
-    "Circuit", "CircuitInstruction", "read_circuit", "write_circuit", "MEASUREMENT_DTYPE", "Engine", "Job",
+    "Circuit", "CircuitInstruction", "read_circuit", "write_circuit", "read_ir", "read_job", "CircuitIR",
+    "MEASUREMENT_DTYPE", "Engine", "Job",
     "Ticket", "default_engine", "Gate", "GateData", "Program", "SimulationOptions",
     "generate_and_write_random_circuit", "generate_random_clifford_circuit", "ExtendedTableau",
     "MeasurementResult", "Tableau",

@@ -226,3 +226,59 @@ def test_autotune_signature_and_cache(tmp_path):
         for v in values:           # joint coordinates list one value per option
             assert len(v if isinstance(k, tuple) else (v,)) == len(ks)
     assert set(autotune.DEFAULTS) == keys
+
+
+def _random_chp_text(rng, n, d, ops):
+    """A .chp file exercising every gate name and alias (mixed case), N1 with and without
+    parameters, and two-index single-qudit lines."""
+    names = ["I", "X", "x_inv", "Z", "Z_INV", "h", "R", "DFT", "H_INV", "h_dag", "P", "S", "phase", "P_INV",
+             "S_INV", "M", "mz", "M_X", "mx", "RESET", "MR", "N1", "noise1"]
+    two = ["CNOT", "sum", "CX", "c", "CNOT_INV", "SUM_INV", "CZ", "cz_inv", "SWAP"]
+    lines = ["random file", "#", f"d {d}"]
+    for _ in range(ops):
+        if rng.random() < 0.35:
+            a, b = rng.choice(n, 2, replace=False)
+            lines.append(f"{rng.choice(two)} {a} {b}")
+        else:
+            name = str(rng.choice(names))
+            line = f"{name} {rng.integers(n)}"
+            if name.upper() in ("N1", "NOISE1") and rng.random() < 0.7:
+                line += f" prob={rng.random():.3f}"
+                if rng.random() < 0.7:
+                    line += f" {rng.choice(['noise_channel', 'channel'])}={rng.choice(['d', 'f', 'p'])}"
+            elif rng.random() < 0.05:
+                line += f" {rng.integers(n)}"
+            lines.append(line)
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_chp_straight_to_ir_equals_circuit_path(tmp_path, seed):
+    """read_ir (file -> IR arrays, no CircuitInstruction objects) against read_circuit +
+    add_gate + the IR builder on generated files and the reference's sample circuits."""
+    from sdim_amd.circuit import ops_to_ir
+    from sdim_amd.circuit_io import read_ir
+    rng = np.random.default_rng(seed)
+    files = [os.path.join(ROOT, "tests", "golden", "epr.chp"), os.path.join(ROOT, "tests", "golden", "css_steane_final.chp")]
+    for k in range(4):
+        p = tmp_path / f"g{k}.chp"
+        p.write_text(_random_chp_text(rng, int(rng.integers(2, 30)), int(rng.choice([2, 3, 5, 7, 11])), 300))
+        files.append(str(p))
+    for f in files:
+        fast = read_ir(f)
+        c = S.read_circuit(f)
+        ir, ch, pr = ops_to_ir(c.operations)
+        assert (fast.num_qudits, fast.dimension) == (c.num_qudits, c.dimension)
+        assert np.array_equal(fast.ir, ir) and np.array_equal(fast.noise_channel, ch)
+        assert np.array_equal(fast.noise_prob, pr), f
+
+
+def test_chp_straight_to_ir_errors(tmp_path):
+    from sdim_amd.circuit_io import read_ir
+    for body, msg in (("FOO 1\n", "Gate FOO not found"), ("H 1 2 3\n", "Unexpected number of arguments"),
+                      ("N1 0 prob=0.1 noise_channel=q\n", "valid noise channel"),
+                      ("N1 0 prob=0.1=2\n", "correct format")):
+        p = tmp_path / "bad.chp"
+        p.write_text("bad\n#\n" + body)
+        with pytest.raises(ValueError, match=msg):
+            read_ir(str(p))

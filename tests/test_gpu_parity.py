@@ -216,3 +216,28 @@ def test_autotune_keeps_records(tmp_path):
     check_ticket(tk, jobs, tableaux=False)
     tk.release()
     eng.close()
+
+
+def test_chp_files_straight_to_device(tmp_path):
+    """.chp files read straight into device gate lists (sdim_amd.circuit_io.read_job, no
+    CircuitInstruction objects): the reference's EPR and Steane samples and generated files
+    with every gate name, alias and N1 parameters, against the oracle."""
+    import os
+    from sdim_amd.circuit_io import read_job
+    from test_api_cpu import _random_chp_text
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    files = [os.path.join(golden, "epr.chp"), os.path.join(golden, "css_steane_final.chp")]
+    rng = np.random.default_rng(99)
+    for k, d in enumerate((2, 3, 5, 7, 11)):
+        p = tmp_path / f"gen{k}.chp"
+        p.write_text(_random_chp_text(rng, int(rng.integers(3, 40)), d, 400))
+        files.append(str(p))
+    jobs = [read_job(f, shots=6, seed=1234 + k, stream=k) for k, f in enumerate(files)]
+    from sdim_amd.engine import Engine
+    eng = Engine()
+    for mode in ("batched", "plain"):
+        print(f"[chp] {len(jobs)} files, {mode}", flush=True)
+        tk = eng.run(jobs, mode=mode)
+        check_ticket(tk, jobs)
+        tk.release()
+    eng.close()
