@@ -97,6 +97,8 @@ const char* sdim_last_error(void);
  *                     byte per qudit and shot; default 65536); larger frames live in device memory
  *   "gang_timeout"    ticks of the 100 MHz constant clock a gang member waits at a barrier before
  *                     the gang gives up and the ticket fails (default 1e9 = 10 s)
+ *   "snapshots"       0 (default), 1: keep the tableau after every measurement event, 2: after every
+ *                     instruction (sdim_fetch_snapshot); such submissions run without gangs
  *   "list_cap"        diagnostics: caps the kernels' LDS lists of touched rows and factor groups
  *                     (0 = their full size); an overflow drops the write and fails the ticket
  */
@@ -173,6 +175,16 @@ int sdim_fetch_records(sdim_ctx* ctx, uint64_t ticket, int32_t job, void* out, i
  */
 int sdim_fetch_tableau(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t shot,
                        int64_t* x, int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph);
+
+/*
+ * Tableau snapshot `index` of (job, shot) in the reference layout (as sdim_fetch_tableau).
+ * Only for tickets submitted with the "snapshots" option: 1 keeps the tableau right after
+ * every measurement event (index = event, before RESET's fix-up; record_tableau,
+ * sdim/program.py:322-324), 2 after every instruction (index = instruction; the
+ * reference's verbose output, sdim/program.py:313-350).
+ */
+int sdim_fetch_snapshot(sdim_ctx* ctx, uint64_t ticket, int32_t job, int64_t index, int64_t shot,
+                        int64_t* x, int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph);
 
 /*
  * Timing and traffic of a finished ticket: device time of the simulation kernels

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libsdim_hip.so")
 EXPORTS = (
     "sdim_ctx_create", "sdim_ctx_destroy", "sdim_last_error", "sdim_set_option", "sdim_get_option",
     "sdim_submit", "sdim_frame_submit", "sdim_simulate_noalloc", "sdim_wait", "sdim_poll", "sdim_release", "sdim_num_measurements",
-    "sdim_fetch_records", "sdim_fetch_tableau", "sdim_ticket_stats", "sdim_replay",
+    "sdim_fetch_records", "sdim_fetch_tableau", "sdim_fetch_snapshot", "sdim_ticket_stats", "sdim_replay",
     "sdim_ticket_counters", "sdim_ticket_info", "sdim_ticket_poison", "sdim_server_create", "sdim_server_submit", "sdim_server_poll",
     "sdim_server_wait", "sdim_server_result", "sdim_server_stats", "sdim_server_destroy",
 )
@@ -69,6 +69,7 @@ def load(path=None):
     L.sdim_num_measurements.argtypes = [vp, u64, i32, ctypes.POINTER(i64)]
     L.sdim_fetch_records.argtypes = [vp, u64, i32, vp, i64]
     L.sdim_fetch_tableau.argtypes = [vp, u64, i32, i64, vp, vp, vp, vp, vp, vp]
+    L.sdim_fetch_snapshot.argtypes = [vp, u64, i32, i64, i64, vp, vp, vp, vp, vp, vp]
     L.sdim_ticket_stats.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                     ctypes.POINTER(u64), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.sdim_replay.argtypes = [vp, u64, i32, i32, ctypes.POINTER(ctypes.c_double)]

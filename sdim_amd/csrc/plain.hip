@@ -87,6 +87,17 @@ __global__ __launch_bounds__(256) void plain_gate_kernel(KParams P, DevOp o) {
   }
 }
 
+// snapshot `o.t` of every item of the launch (one workgroup per item)
+template <int D>
+__global__ __launch_bounds__(256) void plain_snap_kernel(KParams P, DevOp o) {
+  const ItemDesc it = P.items[blockIdx.x];
+  uint8_t* tab = P.arena + it.tab_off;
+  Tab T;
+  T.X = tab; T.Z = tab + (size_t)rows_padded(it.n) * it.rs; T.Ph = tab + 2 * (size_t)rows_padded(it.n) * it.rs;
+  T.n = it.n; T.rs = it.rs; T.S = it.S; T.wofs = 0; T.rs_full = it.rs; T.nw = it.rs / (D == 2 ? 8 : 4); T.il = it.il;
+  snapshot<D, 256>(T, it, P, o.t);
+}
+
 template <int D, int NT>
 __global__ __launch_bounds__(NT) void plain_meas_kernel(KParams P, DevOp o) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -162,6 +173,8 @@ static int plain_op_d(const KParams& P, const DevOp& o, uint32_t n, uint32_t n_i
     hipLaunchKernelGGL((plain_meas_kernel<D, 256>), dim3(n_items), dim3(256), lds, st, P, o);
   } else if (kind == OP_END) {
     hipLaunchKernelGGL(plain_init_kernel<D>, dim3(n_items), dim3(256), 0, st, P);
+  } else if (kind == OP_SNAP) {
+    hipLaunchKernelGGL(plain_snap_kernel<D>, dim3(n_items), dim3(256), 0, st, P, o);
   } else {
     const uint32_t G = 2 * n;
     const uint32_t NWd = (D == 2) ? (G + 63) / 64 : (G + 3) / 4;

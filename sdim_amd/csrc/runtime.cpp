@@ -67,6 +67,7 @@ struct JobPlan {
   int64_t shots = 0;
   int64_t shot_base = 0;     // shot index of the job's first shot (frame sampling starts after the reference shot)
   int64_t n_meas = 0;
+  int64_t n_snap = 0;        // tableau snapshots kept per item (events or instructions)
   std::vector<int32_t> meas_q, meas_round;
   std::vector<uint8_t> meas_kind;   // frame tickets: gate id of each event (14 M, 15 M_X, 16 RESET)
   uint32_t S_item = 1;
@@ -190,6 +191,7 @@ struct sdim_ctx {
   int64_t frame_lds_bytes = 65536;  // frame sampler: LDS for the frames of one workgroup (else device memory)
   int64_t gang_timeout = (int64_t)GANG_TIMEOUT_TICKS;  // barrier wait bound of a gang member (100 MHz ticks)
   int64_t list_cap = 0;       // nonzero: caps the kernels' LDS row / group lists (tests the overflow check)
+  int64_t snapshots = 0;      // tableau snapshots: 1 at every measurement event, 2 after every instruction
   int64_t sweep_rows = 1;     // HBM-resident items: rows in flight per thread in the rowsum sweep (1, 2, 4)
   int64_t seg_batches = 2;    // HBM-resident items: gate batches whose ops are loaded together (1, 2)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
@@ -236,7 +238,7 @@ void ctx_free(sdim_ctx* c, Alloc& a) {
 }
 
 // ---------------------------------------------------------------- lowering
-int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep) {
+int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep, bool snap_every_op = false) {
   const int64_t n = j.n;
   const uint32_t d = (uint32_t)j.d;
   std::vector<int32_t> counts((size_t)n, 0);
@@ -287,6 +289,7 @@ int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep) {
         break;
       }
     }
+    if (snap_every_op) push(OP_SNAP, 0, 0, 0, 0, src, src);   // the tableau after instruction k
   }
   jp.n_meas = (int64_t)jp.meas_q.size();
   return 0;
@@ -360,14 +363,14 @@ void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max,
   size_t i = 0;
   while (i < prims.size()) {
     const Prim& p0 = prims[i];
-    if (p0.kind == OP_MEAS || p0.kind == OP_RESET) {
+    if (p0.kind == OP_MEAS || p0.kind == OP_RESET || p0.kind == OP_SNAP) {
       out.push_back(mkop(p0.kind, 0, 0, p0.q, p0.aux, p0.src));
       i++;
       continue;
     }
     size_t j = i;
     uint32_t nn = 0;
-    while (j < prims.size() && prims[j].kind != OP_MEAS && prims[j].kind != OP_RESET) {
+    while (j < prims.size() && prims[j].kind != OP_MEAS && prims[j].kind != OP_RESET && prims[j].kind != OP_SNAP) {
       if (prims[j].kind == OP_NOISE) {
         if (nn == nn_max) break;
         nn++;
@@ -638,7 +641,7 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
     for (const Prim& p : jp.prims) {
       DevOp o;
       if (p.kind == OP_NOISE) o = mkop(OP_NOISE, p.a, 0, p.q, p.aux, p.src);
-      else if (p.kind == OP_MEAS || p.kind == OP_RESET) o = mkop(p.kind, 0, 0, p.q, p.aux, p.src);
+      else if (p.kind == OP_MEAS || p.kind == OP_RESET || p.kind == OP_SNAP) o = mkop(p.kind, 0, 0, p.q, p.aux, p.src);
       else o = mkop(p.kind, p.a, p.b, p.q, p.t, p.src);
       if (launch_plain_op(Q, o, (uint32_t)jp.n, (uint32_t)jp.d, jp.n_items, Smax, T->stream))
         return fail(SDIM_EHIP, "plain op launch failed");
@@ -758,6 +761,9 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "gang_timeout") {
     if (v < 1) return fail(SDIM_EINVAL, "gang_timeout must be >= 1 tick");
     c->gang_timeout = v;
+  } else if (k == "snapshots") {
+    if (v < 0 || v > 2) return fail(SDIM_EINVAL, "snapshots must be 0, 1 or 2");
+    c->snapshots = v;
   } else if (k == "sweep_rows") {
     if (v != 1 && v != 2 && v != 4) return fail(SDIM_EINVAL, "sweep_rows must be 1, 2 or 4");
     c->sweep_rows = v;
@@ -794,6 +800,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "frame_lds_bytes") *v = c->frame_lds_bytes;
   else if (k == "gang_timeout") *v = c->gang_timeout;
   else if (k == "list_cap") *v = c->list_cap;
+  else if (k == "snapshots") *v = c->snapshots;
   else if (k == "sweep_rows") *v = c->sweep_rows;
   else if (k == "seg_batches") *v = c->seg_batches;
   else return fail(SDIM_EINVAL, "unknown option " + k);
@@ -866,8 +873,9 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     jp.shots = j.shots;
     jp.keep_off = keep.size();
     const size_t keep0 = keep.size();
-    int rc = expand(j, jp, keep);
+    int rc = expand(j, jp, keep, c->snapshots == 2);
     if (rc) return rc;
+    jp.n_snap = c->snapshots == 1 ? jp.n_meas : c->snapshots == 2 ? j.n_ops : 0;
     // keep indices in prims are global; make them circuit-relative
     for (auto& p : jp.prims) if (p.kind == OP_NOISE) p.aux -= (uint32_t)keep0;
 
@@ -880,7 +888,8 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     }
     int cls = -1;
     uint32_t gang_cb = 0;     // gang items: LDS width of a member's columns
-    if (mode == SDIM_MODE_BATCHED && !opts.no_gangs && c->gang_min_n > 0 && j.n >= c->gang_min_n) {
+    // snapshots are taken by one workgroup per item: no gangs then
+    if (mode == SDIM_MODE_BATCHED && !opts.no_gangs && !c->snapshots && c->gang_min_n > 0 && j.n >= c->gang_min_n) {
       // a gang of K workgroups, each owning >= gang_chunk bytes of every row; gang
       // workgroups take the whole LDS of a CU, so one is resident per CU
       const int gnt = (int)c->gang_threads;
@@ -986,7 +995,8 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       it.S = (uint32_t)std::min<int64_t>(S, j.shots - s0);
       it.shot0 = (uint32_t)s0;
       it.rs = rs;
-      it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0) | (ph_lds ? ITEM_PH_LDS : 0);
+      it.flags = (resident ? ITEM_LDS : 0) | (given ? ITEM_INIT_GIVEN : 0) | (ph_lds ? ITEM_PH_LDS : 0) |
+                 (c->snapshots == 1 ? ITEM_SNAP_MEAS : 0);
       it.n_meas = (uint32_t)jp.n_meas;
       it.il = il;
       it.id = (uint32_t)items.size();
@@ -1007,6 +1017,16 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     it.tab_off = off;
     off = align256(off + tableau_bytes(it.n, it.d, it.S));
   }
+  // tableau snapshots after the final tableaux
+  for (const JobPlan& jp : T->jobs) {
+    for (uint32_t i = 0; i < jp.n_items; i++) {
+      ItemDesc& it = items[jp.first_item + i];
+      if (!jp.n_snap) continue;
+      it.snap_off = off;
+      off += (uint64_t)jp.n_snap * snapshot_stride(it.n, it.d, it.S);
+    }
+  }
+  if (off > (64ull << 30)) return fail(SDIM_ENOMEM, "tableau snapshots would need more than 64 GiB");
   st.arena_bytes = std::max<uint64_t>(off, 256);
   // metadata: items, order, ops, keep, queue, counters.  Order: by launch class, longest first inside a class.
   std::vector<uint32_t>& order = st.order;
@@ -1527,6 +1547,32 @@ int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, 
     if (shot >= it.shot0 && shot < (int64_t)it.shot0 + it.S) {
       std::vector<uint8_t> buf(tableau_bytes(it.n, it.d, it.S));
       HIP_OK(hipMemcpy(buf.data(), (uint8_t*)t->arena.p + it.tab_off, buf.size(), hipMemcpyDeviceToHost));
+      unpack_tableau(buf.data(), it.n, it.d, it.il, (uint32_t)(shot - it.shot0), x, z, ph, dx, dz, dph);
+      return 0;
+    }
+  }
+  return fail(SDIM_ENOTFOUND, "shot not found");
+}
+
+int sdim_fetch_snapshot(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t index, int64_t shot, int64_t* x,
+                        int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) {
+  if (!c || !x || !z || !ph || !dx || !dz || !dph) return fail(SDIM_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Ticket* t = find_ticket(c, ticket);
+  if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
+  if (job < 0 || (size_t)job >= t->jobs.size()) return fail(SDIM_ENOTFOUND, "unknown job");
+  int rc = finish(c, t);
+  if (rc) return rc;
+  const JobPlan& jp = t->jobs[(size_t)job];
+  if (!jp.n_snap) return fail(SDIM_EINVAL, "the ticket was submitted without tableau snapshots");
+  if (index < 0 || index >= jp.n_snap) return fail(SDIM_EINVAL, "snapshot index out of range");
+  if (shot < 0 || shot >= jp.shots) return fail(SDIM_EINVAL, "shot out of range");
+  for (uint32_t i = 0; i < jp.n_items; i++) {
+    const ItemDesc& it = t->items[jp.first_item + i];
+    if (shot >= it.shot0 && shot < (int64_t)it.shot0 + it.S) {
+      std::vector<uint8_t> buf(tableau_bytes(it.n, it.d, it.S));
+      const uint8_t* src = (const uint8_t*)t->arena.p + it.snap_off + (uint64_t)index * snapshot_stride(it.n, it.d, it.S);
+      HIP_OK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
       unpack_tableau(buf.data(), it.n, it.d, it.il, (uint32_t)(shot - it.shot0), x, z, ph, dx, dz, dph);
       return 0;
     }

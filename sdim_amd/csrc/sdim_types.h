@@ -39,6 +39,7 @@ enum OpKind : uint32_t {
   OP_CZ = 13,       // H^-1(t) SUM(c,t) H(t):    z[t] += x[c], z[c] += x[t], phase += po x[c] x[t]
   OP_CZI = 14,      // H^-1(t) SUM^-1(c,t) H(t): z[t] -= x[c], z[c] -= x[t], phase -= po x[c] x[t]
   OP_SWAP = 15,     // the nine-primitive prime word: rows q and t exchanged, phases unchanged
+  OP_SNAP = 16,     // copy the item's tableau to snapshot slot t (after instruction t of the circuit)
 };
 
 struct DevOp {
@@ -69,6 +70,8 @@ enum ItemFlags : uint32_t {
   ITEM_LDS = 1u,        // tableau resident in LDS for the whole item
   ITEM_INIT_GIVEN = 2u, // start from the tableau already stored at tab_off (else identity)
   ITEM_PH_LDS = 4u,     // X/Z in HBM, the shots' phase vectors in LDS for the whole item
+  ITEM_SNAP_MEAS = 8u,  // copy the tableau to snapshot slot e right after measurement event e
+                        // (before RESET's fix-up); the snapshots start at snap_off
   // frame items: ITEM_LDS = the frames of the item's shots live in LDS (else at tab_off)
 };
 
@@ -89,7 +92,12 @@ struct ItemDesc {
   uint32_t n_meas;
   uint32_t il;         // X/Z row interleave: 3 (8-row blocks) or 0 (row-major)
   uint32_t id;         // index of the item in its submission (reported with kernel errors)
+  uint64_t snap_off;   // byte offset of the item's tableau snapshots (ITEM_SNAP_MEAS, OP_SNAP), each
+                       // snapshot_stride() bytes in the item's own layout
 };
+
+// bytes between two snapshots of an item's tableau
+static inline __host__ __device__ uint64_t snapshot_stride(uint32_t n, uint32_t d, uint32_t S);
 
 static inline __host__ __device__ uint32_t row_stride(uint32_t n, uint32_t d) {
   uint32_t G = 2 * n;
@@ -111,6 +119,10 @@ static inline __host__ __device__ uint64_t cell(uint32_t r, uint32_t b, uint32_t
 static inline __host__ __device__ uint64_t tableau_bytes(uint32_t n, uint32_t d, uint32_t S) {
   uint64_t rs = row_stride(n, d);
   return 2ull * rows_padded(n) * rs + (d == 2 ? 2ull : 1ull) * S * rs;
+}
+
+static inline __host__ __device__ uint64_t snapshot_stride(uint32_t n, uint32_t d, uint32_t S) {
+  return (tableau_bytes(n, d, S) + 255u) & ~255ull;
 }
 
 }  // namespace sdim

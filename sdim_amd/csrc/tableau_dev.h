@@ -839,6 +839,19 @@ SDIM_DEV void rowsum_phases(const Tab& V, const Scr& S, uint32_t nF, uint32_t e,
     }
 }
 
+// ------------------------------------------------------------------ snapshots
+// The item's whole tableau (X/Z blocks, then every shot's phase vector) copied to snapshot
+// `slot` in the item's own layout (the host converts it like a final tableau).
+template <int D, int NT>
+SDIM_DEV void snapshot(const Tab& T, const ItemDesc& it, const KParams& P, uint32_t slot) {
+  __syncthreads();
+  const uint64_t xz = 2ull * rows_padded(T.n) * T.rs, ph = (D == 2 ? 2ull : 1ull) * T.S * T.rs;
+  uint8_t* dst = P.arena + it.snap_off + (uint64_t)slot * snapshot_stride(T.n, D, T.S);
+  copy16<NT>(dst, T.X, xz);
+  copy16<NT>(dst + xz, T.Ph, ph);
+  __syncthreads();   // the copy reads the tableau before the next step writes it
+}
+
 // ------------------------------------------------------------------ measurement
 // Z-basis measurement of qudit q (tableau_prime.py:262-363) followed, for
 // RESET, by X^((-m) mod d) (program.py:334-339).  Writes one record per shot.
@@ -1143,6 +1156,7 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     SDIM_TADD(P, 5, tDet);
   }
 
+  if (it.flags & ITEM_SNAP_MEAS) snapshot<D, NT>(T, it, P, slot);
   if (reset) {
     SDIM_TMARK(tR);
     __syncthreads();
@@ -1221,6 +1235,9 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
       k = run_segment<D, NT, OPB>(ops, k, T, S, it, P, bytes, lbytes);
     } else if (kind == OP_MEAS || kind == OP_RESET) {
       measure<D, NT, RB>(T, o.q, o.t, o.src, kind == OP_RESET, mk++, S, it, P, bytes);
+      k++;
+    } else if (kind == OP_SNAP) {
+      snapshot<D, NT>(T, it, P, o.t);
       k++;
     } else {
       break;   // OP_END

@@ -108,6 +108,17 @@ class Ticket:
                                                     *[a.ctypes.data for a in arrs]))
         return tuple(arrs)
 
+    def snapshot(self, job=0, index=0, shot=0):
+        """Tableau snapshot `index` of (job, shot): (x, z, ph, dx, dz, dph) in the reference layout
+        (tickets submitted with the engine option snapshots=1: per measurement event, or 2: per
+        instruction)."""
+        n = int(self.jobs[job].n)
+        arrs = [np.zeros((n, n), np.int64), np.zeros((n, n), np.int64), np.zeros(n, np.int64),
+                np.zeros((n, n), np.int64), np.zeros((n, n), np.int64), np.zeros(n, np.int64)]
+        self._check(self.engine.L.sdim_fetch_snapshot(self.engine.ctx, self.id, job, int(index), int(shot),
+                                                      *[a.ctypes.data for a in arrs]))
+        return tuple(arrs)
+
     def stats(self):
         ms = ctypes.c_double(0)
         bh = ctypes.c_uint64(0)

@@ -25,7 +25,7 @@ from typing import Optional
 
 import numpy as np
 
-from .circuit import Circuit, CircuitInstruction, ops_to_ir
+from .circuit import Circuit, ops_to_ir
 from .engine import MEASUREMENT_DTYPE, Job, default_engine
 from .tableau import ExtendedTableau, MeasurementResult, Tableau, is_prime
 
@@ -107,14 +107,14 @@ class Program:
                                                               else random.getrandbits(63))
         self.last_seed = seed
         job = self._job(ops, shots, seed, self.last_stream)
-        tk = engine.run([job], mode=options.mode)
+        # record_tableau: the kernel copies the tableau at every measurement event of this run
+        tk = self._run_with_snapshots(engine, job, options.mode, 1 if options.record_tableau else 0)
         rec = tk.records(0)                       # (events, shots)
         final = tk.tableau(0, shots - 1)
-        tk.release()
-
         snapshots = None
         if options.record_tableau:
-            snapshots = self._measurement_snapshots(engine, ops, shots, seed, options.mode)
+            snapshots = self._measurement_snapshots(tk, n, d, rec.shape[0], shots)
+        tk.release()
         if options.show_gate or options.verbose:
             self._trace(engine, ops, seed, options)
 
@@ -198,25 +198,30 @@ class Program:
                     print(shots_list[0])
         return self.measurement_results
 
-    def _measurement_snapshots(self, engine, ops, shots, seed, mode):
+    @staticmethod
+    def _run_with_snapshots(engine, job, mode, snapshots):
+        """Runs one job with the engine's tableau snapshots set to `snapshots` for this
+        submission (0 none, 1 at every measurement event, 2 after every instruction)."""
+        if not snapshots:
+            return engine.run([job], mode=mode)
+        prev = engine.get_option("snapshots")
+        engine.set_option("snapshots", snapshots)
+        try:
+            return engine.run([job], mode=mode)
+        finally:
+            engine.set_option("snapshots", prev)
+
+    @staticmethod
+    def _measurement_snapshots(tk, n, d, n_events, shots):
         """Tableau right after each measurement event (before RESET's X fix-up), per shot
-        (sdim/program.py:323-324).  Each prefix is re-simulated with the same counter-based
-        draws, so the snapshot belongs to the same trajectory."""
-        n, d = self.stabilizer_tableau.num_qudits, self.stabilizer_tableau.dimension
+        (sdim/program.py:322-324), as copied by the kernel at the event."""
         out = []
-        for k, op in enumerate(ops):
-            if op.gate_id not in (14, 15, 16):
-                continue
-            prefix = list(ops[:k + 1])
-            if op.gate_id == 16:   # measure only: the snapshot precedes the reset fix-up
-                prefix[-1] = CircuitInstruction(op.gate_data, "M", op.qudit_index, None)
-            tk = engine.run([self._job(prefix, shots, seed, self.last_stream)], mode=mode)
+        for ev in range(n_events):
             per_shot = []
             for s in range(shots):
                 t = ExtendedTableau(n, d)
-                t.set_arrays(tk.tableau(0, s))
+                t.set_arrays(tk.snapshot(0, ev, s))
                 per_shot.append(t)
-            tk.release()
             out.append(per_shot)
         return out
 
@@ -228,17 +233,22 @@ class Program:
             print("Initial state")
             self.initial_tableau.print_tableau()
             print("\n")
+        tk = None
+        if options.verbose:
+            # one run of the first shot; the kernel copies the tableau after every instruction
+            tk = self._run_with_snapshots(engine, self._job(ops, 1, seed, self.last_stream, noiseless),
+                                          options.mode, 2)
         for k, op in enumerate(ops):
             if options.show_gate:
                 info = op.target_index if op.target_index is not None else ""
                 print("Time step" if k < length - 1 else "Final step", k, "\t", op.name, op.qudit_index, info)
             if options.verbose:
-                tk = engine.run([self._job(ops[:k + 1], 1, seed, self.last_stream, noiseless)])
                 t = ExtendedTableau(n, d)
-                t.set_arrays(tk.tableau(0, 0))
-                tk.release()
+                t.set_arrays(tk.snapshot(0, k, 0))
                 t.print_tableau()
                 print("\n")
+        if tk is not None:
+            tk.release()
 
     # ------------------------------------------------------------ formats (reference helpers)
     @staticmethod

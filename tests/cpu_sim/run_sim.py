@@ -29,7 +29,7 @@ import numpy as np  # noqa: E402
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--flavor", default="plain")
-    ap.add_argument("--case", default="random", choices=["random", "uneven", "selftest", "noalloc"])
+    ap.add_argument("--case", default="random", choices=["random", "uneven", "selftest", "noalloc", "snapshots"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--count", type=int, default=12)
     ap.add_argument("--hbm", action="store_true", help="random case: also with every tableau in HBM")
@@ -75,6 +75,14 @@ def main():
 
         check_noalloc_against_submit(eng, rng, alloc, to_host, fill)
         print(f"[sim] noalloc ok ({time.time() - t0:.1f} s)", flush=True)
+        eng.close()
+        return
+    if a.case == "snapshots":
+        from snapshot_helpers import check_snapshots
+        rng = np.random.default_rng(a.seed)
+        eng = SimEngine(a.flavor, **opts)
+        check_snapshots(eng, rng, a.count)
+        print(f"[sim] snapshots ok ({time.time() - t0:.1f} s)", flush=True)
         eng.close()
         return
     if a.case == "random":

@@ -45,6 +45,14 @@ def test_cpu_build_matches_oracle():
     assert r.returncode == 0, out[-4000:]
 
 
+def test_cpu_build_snapshots_and_noalloc():
+    """Tableau snapshots (per measurement event, per instruction) against prefix runs, and the
+    allocation-free entry point against sdim_submit, on the CPU build."""
+    for case in ("snapshots", "noalloc"):
+        r, out = _run("plain", "--case", case, "--count", "2", timeout=600)
+        assert r.returncode == 0, out[-4000:]
+
+
 def test_address_sanitizer_random_circuits():
     r, out = _run("asan", "--case", "random", "--seed", "12", "--count", "4", "--hbm")
     assert r.returncode == 0 and ASAN_REPORT not in out, out[-6000:]

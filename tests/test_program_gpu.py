@@ -271,3 +271,22 @@ def test_initial_tableau_and_extended_tableau_methods():
     assert got == [(q, r, s, d, v) for (q, r, s, d, v) in exp]
     mr = t.measure(1)
     assert isinstance(mr, S.MeasurementResult) and mr.qudit_index == 1
+
+
+def test_kernel_snapshots_equal_prefix_runs():
+    """The kernel's tableau snapshots (per measurement event for record_tableau, per instruction
+    for verbose) equal the final tableaux of the circuits cut at those points."""
+    from snapshot_helpers import check_snapshots
+    from sdim_amd.engine import Engine
+    eng = Engine()
+    print("[snapshots] random circuits, batched and plain", flush=True)
+    check_snapshots(eng, np.random.default_rng(808), count=3)
+    eng.close()
+
+
+def test_verbose_prints_every_instruction(capsys):
+    c = S.Circuit(2, 3)
+    c.add_gate("H", 0); c.add_gate("CNOT", 0, 1); c.add_gate("M", 1); c.add_gate("RESET", 1)
+    S.Program(c, seed=3).simulate(shots=1, verbose=True, show_gate=True)
+    out = capsys.readouterr().out
+    assert out.count("Time step") == 3 and out.count("Final step") == 1 and "Initial state" in out
