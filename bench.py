@@ -196,17 +196,25 @@ def serving(eng, n_requests, rate, budget_gb, seed=5):
     rng = np.random.default_rng(seed)
     reqs = [W.serving_request(rng, k, seed=seed) for k in range(n_requests)]
     gate_shots = W.count_gate_shots(reqs)
+    # burst capacity: the first requests submitted at once, completion rate measured (also the
+    # calibration of the Poisson rate, 60% of it, when no rate is given)
+    srv = Server(eng, hbm_budget_bytes=int(budget_gb * (1 << 30)))
+    burst = reqs[:min(100, n_requests)]
+    t0 = time.perf_counter()
+    ids = [srv.submit(r) for r in burst]
+    for rid in ids:
+        srv.wait(rid)
+        srv.result(rid)
+    burst_s = time.perf_counter() - t0
+    srv.close()
+    burst_gs = W.count_gate_shots(burst)
     if rate <= 0:
-        # burst calibration: everything submitted at once, completion rate measured
-        srv = Server(eng, hbm_budget_bytes=int(budget_gb * (1 << 30)))
-        burst = reqs[:min(60, n_requests)]
-        t0 = time.perf_counter()
-        ids = [srv.submit(r) for r in burst]
-        for rid in ids:
-            srv.wait(rid)
-            srv.result(rid)
-        rate = 0.6 * len(burst) / (time.perf_counter() - t0)
-        srv.close()
+        rate = 0.6 * len(burst) / burst_s
+    # where the gate-shots are, by size band: a request below gang_min_n runs on one workgroup
+    # (LDS-resident while its tableau fits), one at or above it on a gang of workgroups
+    def share(lo, hi):
+        return round(sum(W.count_gate_shots([r]) for r in reqs if lo <= r.n < hi) / gate_shots, 4)
+    gmin = eng.get_option("gang_min_n")
     srv = Server(eng, hbm_budget_bytes=int(budget_gb * (1 << 30)))
     ids = poisson_run(srv, lambda k: reqs[k], rate, n_requests, seed=seed)
     for rid in ids:
@@ -218,6 +226,13 @@ def serving(eng, n_requests, rate, budget_gb, seed=5):
             "p99_latency_s": st["p99_s"], "gate_shots": gate_shots,
             "aggregate_gate_shots_per_s": st["gate_shots"] / st["span_s"] if st["span_s"] else None,
             "span_s": st["span_s"], "hbm_budget_gb": budget_gb,
+            "burst": {"requests": len(burst), "seconds": round(burst_s, 4),
+                      "requests_per_s": round(len(burst) / burst_s, 2),
+                      "gate_shots_per_s": burst_gs / burst_s},
+            "gate_shot_share": {"n<256": share(0, 256), f"256<=n<{gmin} (one workgroup)": share(256, gmin),
+                                f"n>={gmin} (gang)": share(gmin, 1 << 40)},
+            "requests_by_band": [sum(1 for r in reqs if r.n < 256), sum(1 for r in reqs if 256 <= r.n < gmin),
+                                 sum(1 for r in reqs if r.n >= gmin)],
             "workload": "Poisson arrivals; n log-U[8,2048], d in {2,3,5,7,11}, depth 8n, shots log-U[1,4096]"}
 
 

@@ -68,3 +68,58 @@ def test_budget_admission():
     with pytest.raises(ValueError):
         tiny.wait(rid, timeout=60)
     tiny.close()
+
+
+def test_serving_mix_at_full_distribution():
+    """Requests from the serving configuration's own distribution (n log-uniform in [8, 2048],
+    d in {2,3,5,7,11}, depth 8n, shots log-uniform in [1, 4096]) plus forced corners: a
+    request on the gang path (n = 2048, d = 11), 4096-shot requests and d = 11.  Each request's
+    records are identical alone and in the mix; the oracle checks the first shots of every
+    request it finishes quickly; the gang request also equals the one-workgroup path."""
+    from sdim_amd.engine import Engine, Job
+    from sdim_amd.serve import Server
+    rng = np.random.default_rng(2025)
+    reqs = [W.serving_request(rng, rid, seed=11) for rid in range(24)]
+    corner = np.random.default_rng(7)
+    for n, d, shots in ((2048, 11, 8), (512, 7, 4096), (8, 11, 4096), (1600, 2, 16)):
+        ir = W.random_clifford_ir(corner, n, d, 8 * n, [W.H, W.P, W.CNOT, W.X, W.Z])
+        reqs.append(Job(n, d, ir, shots=shots, seed=11, stream=100 + len(reqs)))
+    eng = Engine()
+    srv = Server(eng, hbm_budget_bytes=200 << 30, max_inflight=4, max_batch=16)
+    print(f"[serve] {len(reqs)} requests, n up to {max(r.n for r in reqs)}, shots up to {max(r.shots for r in reqs)}",
+          flush=True)
+    ids = []
+    for k, r in enumerate(reqs):
+        ids.append(srv.submit(r))
+        if k % 5 == 2:
+            time.sleep(0.003)   # arrivals while earlier requests are on the device
+    got = []
+    for rid in ids:
+        srv.wait(rid, timeout=300)
+        got.append(srv.result(rid)[0])
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    for k, (r, rec) in enumerate(zip(reqs, got)):
+        tk = eng.run([r])
+        if r.n >= 1536:
+            assert tk.info("gang_items") == 1, "the large request should run on a gang"
+        assert np.array_equal(tk.records(0), rec), f"request {k} (n={r.n}, d={r.d}, shots={r.shots}) differs alone"
+        tk.release()
+        if r.n <= 160:
+            s = min(r.shots, 16)
+            o, _ = c_oracle.simulate(r.n, r.d, np.asarray(r.ir), shots=s, seed=r.seed, stream=r.stream)
+            vals = np.transpose(rec['measurement_value'][:, :s])
+            dets = np.transpose(rec['deterministic'][:, :s]).astype(np.int64)
+            assert np.array_equal(vals, o[:, :, 4]) and np.array_equal(dets, o[:, :, 3]), f"request {k} vs oracle"
+    big = next(k for k, r in enumerate(reqs) if r.n == 2048)
+    r = reqs[big]
+    o, _ = c_oracle.simulate(r.n, r.d, np.asarray(r.ir), shots=1, seed=r.seed, stream=r.stream)
+    assert np.array_equal(got[big]['measurement_value'][:, 0], o[0, :, 4])
+    assert np.array_equal(got[big]['deterministic'][:, 0].astype(np.int64), o[0, :, 3])
+    one_wg = Engine(gang_min_n=0)
+    tk = one_wg.run([reqs[big]])
+    assert tk.info("gang_items") == 0 and np.array_equal(tk.records(0), got[big])
+    tk.release()
+    one_wg.close()
+    eng.close()
