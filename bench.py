@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--shots-per-item", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the leg on the reference's published 100,000-gate workload")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-per-shot", action="store_true",
@@ -234,6 +236,56 @@ def serving(eng, n_requests, rate, budget_gb, seed=5):
             "requests_by_band": [sum(1 for r in reqs if r.n < 256), sum(1 for r in reqs if 256 <= r.n < gmin),
                                  sum(1 for r in reqs if r.n >= gmin)],
             "workload": "Poisson arrivals; n log-U[8,2048], d in {2,3,5,7,11}, depth 8n, shots log-U[1,4096]"}
+
+
+# wall seconds the reference publishes for one 100,000-gate circuit, one shot
+# (tests/testdata/simulation_data.json:23-26, tableau_times at n = 2 and n = 14)
+PUBLISHED_S = {(2, 2): 0.13554, (2, 14): 0.60069, (3, 2): 0.13753, (3, 14): 0.60742,
+               (5, 2): 0.13361, (5, 14): 0.61090, (7, 2): 0.13227, (7, 14): 0.61197}
+
+
+def published_workload(eng, steps, cpu_seconds):
+    """The reference's published timing workload: d in {2,3,5,7} x n in 2..14, one circuit of
+    100,000 gates (CNOT 40% / H 30% / P 30%) each, one shot, no measurement.  All 52 circuits
+    batched in one submission; n = 2 and n = 14 per d alone (batched and plain) beside the
+    published seconds; the CPU port on one circuit."""
+    from sdim_amd import workloads as W
+    wl = W.published_random()
+    tk = eng.submit(wl.jobs, "batched").wait()
+    tk.replay(1)
+    t0 = time.perf_counter()
+    tk.replay(steps)
+    t_all = (time.perf_counter() - t0) / steps
+    tk.release()
+    rows = []
+    for (d, n), pub in PUBLISHED_S.items():
+        job = next(j for j in wl.jobs if j.d == d and j.n == n)
+        tb = eng.submit([job], "batched").wait()
+        tb.replay(1)
+        t0 = time.perf_counter()
+        tb.replay(steps)
+        b_s = (time.perf_counter() - t0) / steps
+        tb.release()
+        tp = eng.submit([job], "plain").wait()
+        t0 = time.perf_counter()
+        tp.replay(1)
+        p_s = time.perf_counter() - t0
+        tp.release()
+        rows.append({"d": d, "n": n, "published_s": pub, "batched_alone_s": round(b_s, 5),
+                     "plain_s": round(p_s, 4), "batched_vs_published": round(pub / b_s, 1)})
+    out = {"workload": wl.name, "description": wl.description, "circuits": len(wl.jobs),
+           "gate_shots": wl.gate_shots, "value": wl.gate_shots / t_all, "unit": "gate-shots/s",
+           "ms_per_step": 1e3 * t_all, "per_circuit": rows}
+    if cpu_seconds > 0:
+        from oracle import program as OP
+        job = next(j for j in wl.jobs if j.d == 2 and j.n == 14)
+        ops = [tuple(r) for r in np.asarray(job.ir).tolist()]
+        t0 = time.perf_counter()
+        OP.simulate(job.n, job.d, ops, shots=1, seed=job.seed, stream=job.stream, keep_tableaux=False)
+        c_s = time.perf_counter() - t0
+        out["cpu_port"] = {"seconds": round(c_s, 3), "d": 2, "n": 14, "cores": 1, "kind": "port",
+                           "sample": "oracle/program.py on the d=2, n=14 circuit, 1 shot"}
+    return out
 
 
 def large_tableau(eng, steps, warmup):
@@ -489,6 +541,10 @@ def main():
 
     if not args.no_frame and args.workload == "uneven":
         out["frame"] = frame_sampler(eng, max(1, min(args.steps, 10)), 1, args.cpu_seconds)
+
+    if not args.no_published and args.workload == "uneven":
+        out["published_workload"] = published_workload(eng, max(1, min(args.steps, 5)),
+                                                        0 if args.no_cpu else args.cpu_seconds)
 
     if not args.no_serve:
         out["serving"] = serving(eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)

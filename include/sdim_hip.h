@@ -99,6 +99,9 @@ const char* sdim_last_error(void);
  *                     the gang gives up and the ticket fails (default 1e9 = 10 s)
  *   "snapshots"       0 (default), 1: keep the tableau after every measurement event, 2: after every
  *                     instruction (sdim_fetch_snapshot); such submissions run without gangs
+ *   "serial_max"      gate segments: a run of two or more consecutive layers of at most this many
+ *                     gates (gates on disjoint qudits) becomes one serial layer that each column word
+ *                     applies in program order without a barrier per layer (0 = off; default 4)
  *   "list_cap"        diagnostics: caps the kernels' LDS lists of touched rows and factor groups
  *                     (0 = their full size); an overflow drops the write and fails the ticket
  */

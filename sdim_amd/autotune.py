@@ -34,13 +34,15 @@ SPACE = [
     (("hbm_threads", "hbm_per_cu"), ((256, 4), (512, 4), (512, 2), (256, 2), (1024, 1), (128, 8))),
     ("sweep_rows", (1, 2, 4)),
     ("seg_batches", (2, 1)),
+    ("serial_max", (4, 0, 2, 8, 16)),
     ("shots_per_item", (0, 1, 2, 4, 8, 16, 32)),
     ("lds_bytes", (163584, 131072, 98304, 65536)),
     ("gang_min_n", (1536, 1024, 2048, 4096, 0)),
     ("gang_chunk", (128, 256, 64)),
     ("gang_threads", (512, 256, 1024)),
 ]
-DEFAULTS = {"hbm_threads": 256, "hbm_per_cu": 4, "sweep_rows": 1, "seg_batches": 2, "shots_per_item": 0,
+DEFAULTS = {"hbm_threads": 256, "hbm_per_cu": 4, "sweep_rows": 1, "seg_batches": 2, "serial_max": 4,
+            "shots_per_item": 0,
             "lds_bytes": 163584, "gang_min_n": 1536, "gang_chunk": 128, "gang_threads": 512}
 # options that only matter for some workloads
 _GANG = ("gang_min_n", "gang_chunk", "gang_threads")

@@ -194,6 +194,7 @@ struct sdim_ctx {
   int64_t snapshots = 0;      // tableau snapshots: 1 at every measurement event, 2 after every instruction
   int64_t sweep_rows = 1;     // HBM-resident items: rows in flight per thread in the rowsum sweep (1, 2, 4)
   int64_t seg_batches = 2;    // HBM-resident items: gate batches whose ops are loaded together (1, 2)
+  int64_t serial_max = 4;     // runs of >= 2 layers of at most this many gates become one serial layer (0: off)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
   std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
 };
@@ -355,8 +356,11 @@ int frame_expand(const sdim_job& j, const uint8_t* ref, JobPlan& jp, std::vector
 }
 
 // segments + ASAP layers for the batch kernel
+// A run of at least two consecutive layers of at most serial_max ops each becomes one serial
+// layer (their ops in layer order, which respects every dependency): the kernel applies it
+// without a barrier per layer.  serial_max = 0 keeps every layer parallel.
 void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max, uint64_t keep_base,
-                    std::vector<DevOp>& out) {
+                    uint32_t serial_max, std::vector<DevOp>& out) {
   (void)keep_base;
   std::vector<int32_t> last(n, -1);
   std::vector<uint32_t> touched;
@@ -397,6 +401,24 @@ void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max,
     }
     for (uint32_t q : touched) last[q] = -1;
     touched.clear();
+    std::vector<std::vector<uint32_t>> merged;
+    std::vector<uint8_t> serial;
+    for (size_t a = 0; a < layers.size();) {
+      size_t b = a;
+      while (serial_max && b < layers.size() && layers[b].size() <= serial_max) b++;
+      if (b - a >= 2) {
+        std::vector<uint32_t> run;
+        for (size_t l = a; l < b; l++) run.insert(run.end(), layers[l].begin(), layers[l].end());
+        merged.push_back(std::move(run));
+        serial.push_back(1);
+        a = b;
+      } else {
+        merged.push_back(std::move(layers[a]));
+        serial.push_back(0);
+        a++;
+      }
+    }
+    layers.swap(merged);
     out.push_back(mkop(OP_SEG, 0, 0, (uint32_t)layers.size(), nn, 0));
     std::vector<uint32_t> slot_of(j - i, 0);
     uint32_t s = 0;
@@ -406,8 +428,9 @@ void build_segments(const std::vector<Prim>& prims, uint32_t n, uint32_t nn_max,
         out.push_back(mkop(OP_NOISE, prims[k].a, 0, prims[k].q, prims[k].aux, prims[k].src));
       }
     }
-    for (auto& L : layers) {
-      out.push_back(mkop(OP_LAYER, 0, 0, (uint32_t)L.size(), 0, 0));
+    for (size_t li = 0; li < layers.size(); li++) {
+      const auto& L = layers[li];
+      out.push_back(mkop(OP_LAYER, 0, 0, (uint32_t)L.size(), serial[li], 0));
       for (uint32_t k : L) {
         const Prim& p = prims[k];
         if (p.kind == OP_NOISE) {
@@ -770,6 +793,9 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "seg_batches") {
     if (v != 1 && v != 2) return fail(SDIM_EINVAL, "seg_batches must be 1 or 2");
     c->seg_batches = v;
+  } else if (k == "serial_max") {
+    if (v < 0 || v > 65535) return fail(SDIM_EINVAL, "serial_max must be in [0, 65535]");
+    c->serial_max = v;
   } else if (k == "list_cap") {
     if (v < 0 || v > 65535) return fail(SDIM_EINVAL, "list_cap must be in [0, 65535]");
     c->list_cap = v;
@@ -803,6 +829,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "snapshots") *v = c->snapshots;
   else if (k == "sweep_rows") *v = c->sweep_rows;
   else if (k == "seg_batches") *v = c->seg_batches;
+  else if (k == "serial_max") *v = c->serial_max;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -971,7 +998,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       while (nn_max < 65535 && noise_bytes(nn_max + 1, w, S) <= room) nn_max++;
       if (has_noise && nn_max == 0) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
       jp.ops_off = (uint32_t)ops.size();
-      build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, ops);
+      build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, (uint32_t)c->serial_max, ops);
       jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
     }
     // items

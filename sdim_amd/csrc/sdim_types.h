@@ -33,7 +33,8 @@ enum OpKind : uint32_t {
   OP_MEAS = 8,      // t = measurement slot
   OP_RESET = 9,     // t = measurement slot; then X^((-m) mod d)
   OP_SEG = 10,      // gate segment header: q = #layers, t = #noise ops in the segment
-  OP_LAYER = 11,    // layer header: q = #ops in the layer
+  OP_LAYER = 11,    // layer header: q = #ops in the layer; t = 1: a serial layer (ops in program
+                    // order that may share qudits, applied one after another)
   OP_END = 12,
   // closed forms of the reference's words (sdim/tableau/tableau_gates.py), equal on every column:
   OP_CZ = 13,       // H^-1(t) SUM(c,t) H(t):    z[t] += x[c], z[c] += x[t], phase += po x[c] x[t]

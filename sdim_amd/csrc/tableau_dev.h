@@ -441,9 +441,64 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
   uint64_t b0 = 0, b1 = 0;
 
   for (uint32_t l = 0; l < nl; l++) {
-    const uint32_t L = ops[kl].q;
+    const DevOp lh = ops[kl];
+    const uint32_t L = lh.q;
     const DevOp* lops = ops + kl + 1;
-    if (active) {
+    // a serial layer holds ops in program order that may share qudits: the thread of every
+    // word applies them one after another (a word's generators are independent of the other
+    // words', so no barrier is needed between them).  Each wave that holds such threads
+    // fetches 64 ops per global round trip, one per lane, the next 64 in flight, and
+    // broadcasts them with readlane.
+    const bool ser = lh.t != 0;
+    if (ser) {
+      const uint32_t nwv = ((NWd < NT ? NWd : NT) + 63) / 64;
+      if (tid / 64 < nwv) {
+        const uint32_t lane = tid & 63;
+        for (uint32_t wb = 0; wb < NWd; wb += NT) {
+          const uint32_t w = wb + tid;
+          const bool mine = w < NWd;
+          if (many) { dl = 0; b0 = 0; b1 = 0; }
+          uint32_t nw0 = 0, nq = 0, ntg = 0;
+          if (lane < L) { const DevOp& src = lops[lane]; nw0 = src.w0; nq = src.q; ntg = src.t; }
+          for (uint32_t c0 = 0; c0 < L; c0 += 64) {
+            const uint32_t cw0 = nw0, cq = nq, ctg = ntg;
+            if (c0 + 64 + lane < L) { const DevOp& src = lops[c0 + 64 + lane]; nw0 = src.w0; nq = src.q; ntg = src.t; }
+            const uint32_t cnt = (L - c0 < 64) ? L - c0 : 64;
+            for (uint32_t b = 0; b < cnt; b++) {
+              DevOp o;
+              o.w0 = (uint32_t)__builtin_amdgcn_readlane((int)cw0, (int)b);
+              o.q = (uint32_t)__builtin_amdgcn_readlane((int)cq, (int)b);
+              o.t = (uint32_t)__builtin_amdgcn_readlane((int)ctg, (int)b);
+              o.src = 0;
+              if (!mine) continue;
+              if constexpr (BIN) {
+                const uint64_t xq = *w64(V.X, o.q, w, V.rs, V.il), zq = *w64(V.Z, o.q, w, V.rs, V.il);
+                uint64_t xt = 0, zt = 0;
+                if (two_rows(o)) { xt = *w64(V.X, o.t, w, V.rs, V.il); zt = *w64(V.Z, o.t, w, V.rs, V.il); }
+                gb += apply_bin(o, xq, zq, xt, zt, V, w, b0, b1, S);
+              } else {
+                const uint32_t xq = *w32(V.X, o.q, w, V.rs, V.il), zq = *w32(V.Z, o.q, w, V.rs, V.il);
+                uint32_t xt = 0, zt = 0;
+                if (two_rows(o)) { xt = *w32(V.X, o.t, w, V.rs, V.il); zt = *w32(V.Z, o.t, w, V.rs, V.il); }
+                gb += apply_odd<D>(o, xq, zq, xt, zt, V, w, dl, S);
+              }
+            }
+          }
+          if (many && mine) {
+            if constexpr (BIN) {
+              uint64_t* p0 = (uint64_t*)S.DW + V.wofs + w;
+              uint64_t* p1 = (uint64_t*)(S.DW + lw) + V.wofs + w;
+              const uint64_t a0 = *p0, a1 = *p1;
+              *p1 = a1 ^ b1 ^ (a0 & b0);
+              *p0 = a0 ^ b0;
+            } else {
+              uint32_t* pd = (uint32_t*)S.DW + V.wofs + w;
+              *pd = addm<D>(*pd, dl);
+            }
+          }
+        }
+      }
+    } else if (active) {
       for (uint32_t w = my_w; w < NWd; w += NT) {
         if (many) { dl = 0; b0 = 0; b1 = 0; }
         // the ops of OB consecutive batches are loaded first (one global round trip), then the

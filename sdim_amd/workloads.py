@@ -112,6 +112,25 @@ def large_tableau(seed=4, n=8192, d=5, shots=32, layers=16, meas_per_layer=64):
                     "d=5 n=8192 4n {H,P,SUM} + 16x(n/2 SUM + 64 M), 32 shots")
 
 
+def published_random(seed=6, gates=100_000, dims=(2, 3, 5, 7), ns=tuple(range(2, 15))):
+    """The reference's published timing workload (tests/testdata/simulation_data.json:2-28):
+    for every d in {2,3,5,7} and n in 2..14 one random circuit of 100,000 gates drawn
+    CNOT 40% / H 30% / P 30%, no measurement, one shot.  Pairs and qudits are drawn as in
+    random_clifford_ir."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for d in dims:
+        for n in ns:
+            g = rng.choice(np.array([CNOT, H, P], np.int64), size=gates, p=[0.4, 0.3, 0.3])
+            q = rng.integers(n, size=gates)
+            t = rng.integers(max(n - 1, 1), size=gates)
+            t = np.where(g == CNOT, t + (t >= q), -1)
+            jobs.append(Job(n, d, np.stack([g, q, t], axis=1).astype(np.int64), shots=1, seed=seed,
+                            stream=len(jobs)))
+    return Workload("published_random_1e5", jobs,
+                    "d in {2,3,5,7} x n in 2..14: 100,000 gates CNOT 40% / H 30% / P 30%, no measurement, 1 shot")
+
+
 def serving_request(rng, rid, seed=5, n_lo=8, n_hi=2048, dims=(2, 3, 5, 7, 11), shots_hi=4096):
     """One serving request: n log-uniform in [8, 2048], d in {2,3,5,7,11}, depth 8n, shots log-uniform [1, 4096]."""
     n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))

@@ -134,6 +134,8 @@ inline uint32_t simt_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
 }
 #define __builtin_amdgcn_perm(hi, lo, sel) simt_perm((hi), (lo), (sel))
 #define __builtin_amdgcn_s_sleep(n) ::simt::yield()
+// readlane: every live lane of the wave takes lane l's value (l is wave-uniform)
+#define __builtin_amdgcn_readlane(v, l) __shfl((int)(v), (int)(l))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 
 // ------------------------------------------------------------------ atomics (device semantics: relaxed)

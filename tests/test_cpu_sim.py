@@ -53,6 +53,19 @@ def test_cpu_build_snapshots_and_noalloc():
         assert r.returncode == 0, out[-4000:]
 
 
+def test_serial_layers_match_oracle():
+    """Gate segments with runs of small layers applied serially (serial_max): random circuits
+    in LDS and HBM, and uneven-batch circuits with every run merged, against the oracle;
+    the random circuits again under AddressSanitizer."""
+    r, out = _run("plain", "--case", "random", "--seed", "21", "--count", "8", "--hbm", "--opt", "serial_max=8")
+    assert r.returncode == 0, out[-4000:]
+    r, out = _run("plain", "--case", "uneven", "--first", "3", "--stride", "512", "--check", "8",
+                  "--opt", "serial_max=1000")
+    assert r.returncode == 0, out[-4000:]
+    r, out = _run("asan", "--case", "random", "--seed", "24", "--count", "3", "--hbm", "--opt", "serial_max=6")
+    assert r.returncode == 0 and ASAN_REPORT not in out, out[-6000:]
+
+
 def test_address_sanitizer_random_circuits():
     r, out = _run("asan", "--case", "random", "--seed", "12", "--count", "4", "--hbm")
     assert r.returncode == 0 and ASAN_REPORT not in out, out[-6000:]

@@ -54,7 +54,8 @@ def test_gang_measure_all_with_noise(gang_engine, d):
 
 @pytest.mark.parametrize("d", [2, 5])
 def test_gang_sizes_agree_with_one_workgroup(d):
-    """The same circuit on gangs of 2, 8 and 64 members and on one workgroup."""
+    """The same circuit on gangs of 2, 8 and 64 members (and of 2 members of 256 threads with
+    serial gate layers) and on one workgroup."""
     from sdim_amd.engine import Engine
     rng = np.random.default_rng(6000 + d)
     n = 1100
@@ -62,7 +63,9 @@ def test_gang_sizes_agree_with_one_workgroup(d):
     rec_o, _ = oracle_of(job, 0)
     runs = []
     for opts in (dict(gang_min_n=0), dict(gang_min_n=256, gang_chunk=16, gang_max=2),
-                 dict(gang_min_n=256, gang_chunk=16, gang_max=8), dict(gang_min_n=256, gang_chunk=16, gang_max=64)):
+                 dict(gang_min_n=256, gang_chunk=16, gang_max=8), dict(gang_min_n=256, gang_chunk=16, gang_max=64),
+                 # serial layers; for d = 5 a member owns more column words than it has threads
+                 dict(gang_min_n=256, gang_chunk=16, gang_max=2, gang_threads=256, serial_max=1000)):
         print(f"[gang] sizes d={d} opts={opts}", flush=True)
         eng = Engine(**opts)
         tk = eng.run([job])

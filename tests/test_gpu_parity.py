@@ -31,11 +31,13 @@ def test_random_circuits_small(engine, mode, d):
 
 
 @pytest.mark.parametrize("opts", [dict(), dict(allow_lds=0), dict(shots_per_item=1), dict(threads=256),
-                                  dict(threads=512, allow_lds=0)])
+                                  dict(threads=512, allow_lds=0), dict(serial_max=0),
+                                  dict(serial_max=1000, allow_lds=0)])
 def test_medium_circuits_engine_options(opts):
+    import zlib
     from sdim_amd.engine import Engine
     eng = Engine(**opts)
-    rng = np.random.default_rng(hash(tuple(sorted(opts.items()))) & 0xFFFF)
+    rng = np.random.default_rng(zlib.crc32(repr(sorted(opts.items())).encode()) & 0xFFFF)
     jobs = []
     for i, (n, d) in enumerate([(40, 2), (70, 3), (33, 5), (50, 7), (21, 11), (130, 2), (96, 3)]):
         jobs.append(clifford_job(rng, n, d, 4 * n, shots=3, stream=i, noise_p=0.05 if i % 2 else 0.0))
@@ -179,6 +181,39 @@ def test_plain_small_configuration(mode):
     tk = eng.run(wl.jobs, mode=mode)
     check_job(tk, 0, wl.jobs[0], tableaux=True, shots=(0, 1, 511, 1023))
     tk.release()
+    eng.close()
+
+
+def test_published_random_workload():
+    """The reference's published timing workload (100,000 gates CNOT 40% / H 30% / P 30%,
+    d in {2,3,5,7}, n in 2..14, one shot, no measurement): the final tableau of all 52
+    circuits batched, and of n = 2 and n = 14 per d on the plain path, against the oracle."""
+    from sdim_amd import workloads as W
+    from sdim_amd.engine import Engine
+    from parity_helpers import oracle_of
+    wl = W.published_random()
+    eng = Engine(serial_max=0)
+    print("[parity-config] published_random batched", flush=True)
+    tk = eng.run(wl.jobs)
+    print("[parity-config] published_random batched, serial layers", flush=True)
+    es = Engine(serial_max=1000)
+    ts = es.run(wl.jobs)
+    sub = [i for i, j in enumerate(wl.jobs) if j.n in (2, 14)]
+    print("[parity-config] published_random plain", flush=True)
+    tp = eng.run([wl.jobs[i] for i in sub], mode="plain")
+    for i, job in enumerate(wl.jobs):
+        _, tabs = oracle_of(job, 1)
+        for name, a, b in zip(("x", "z", "ph", "dx", "dz", "dph"), tk.tableau(i, 0), tabs[0]):
+            assert np.array_equal(a, b), f"tableau {name} differs, circuit {i} (n={job.n}, d={job.d})"
+        for name, a, b in zip(("x", "z", "ph", "dx", "dz", "dph"), ts.tableau(i, 0), tabs[0]):
+            assert np.array_equal(a, b), f"serial-layer tableau {name} differs, circuit {i} (n={job.n}, d={job.d})"
+        if i in sub:
+            for name, a, b in zip(("x", "z", "ph", "dx", "dz", "dph"), tp.tableau(sub.index(i), 0), tabs[0]):
+                assert np.array_equal(a, b), f"plain tableau {name} differs, circuit {i} (n={job.n}, d={job.d})"
+    tk.release()
+    tp.release()
+    ts.release()
+    es.close()
     eng.close()
 
 
