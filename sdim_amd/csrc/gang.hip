@@ -194,13 +194,20 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
   // ---- A: own 16-byte groups of row q -> pivot candidate, f, groups with f != 0
   rowq_groups<D, NT>(V, w0, q, S, M, nullptr, bytes);
   __syncthreads();
-  if (tid == 0 && M[0] != INT_MAX)
-    __hip_atomic_fetch_min(&Gc.sy->piv[mk & 1], M[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // the gang's pivot is the least candidate column; its code carries x[q, p] in the low four
+  // bits so that every member knows the exponent e right after the barrier
+  if (tid == 0 && M[0] != INT_MAX) {
+    const uint32_t pc = (uint32_t)M[0];
+    uint32_t xqp = 1;
+    if constexpr (!BIN) xqp = (D - S.F[pc - c_lo]) % D;      // F = -x[q, .] on own columns
+    __hip_atomic_fetch_min(&Gc.sy->piv[mk & 1], (int)(pc * 16u + xqp), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   SDIM_TADD(P, 0, tA);
   SDIM_TMARK(tG1);
   if (!gang_barrier(Gc, s_ok)) return false;                                // G1
   SDIM_TADD(P, 1, tG1);
-  const int piv = __hip_atomic_load(&Gc.sy->piv[mk & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int pcode = __hip_atomic_load(&Gc.sy->piv[mk & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int piv = pcode == INT_MAX ? INT_MAX : pcode >> 4;
   if (tid == 0) {
     Mn[0] = INT_MAX; Mn[1] = 0; Mn[2] = 0; Mn[3] = 0;
     // the other slot is next written in measurement mk + 1, after this one's last barrier
@@ -212,19 +219,61 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     const bool own_p = p >= c_lo && p < c_hi;
     const bool own_g = n + p >= c_lo && n + p < c_hi;
     uint8_t* stg = Gc.stage[mk & 1];
+    // exponentiation of stabilizer p by e = x[q,p]^-1 (tableau_prime.py:278-282, 365-380)
+    uint32_t e = 1;
+    if constexpr (!BIN) {
+      const uint32_t xv = (uint32_t)pcode & 15u;
+      #pragma unroll
+      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
+    }
     SDIM_TMARK(tB);
     // ---- B: every member packs its slice of rows of column p (x | z << 4 per row) and of the
-    // pivot phases into the staging buffer: one round of strided loads per member
+    // pivot phases into the staging buffer: one round of strided loads per member.  At the
+    // same rows it gives columns p and n+p their final values (destabilizer n+p becomes e
+    // times stabilizer p at every row, stabilizer p becomes Z_q at the touched rows), so that
+    // no member rewrites n scattered rows alone; the owners' sweep keeps both columns, f
+    // being 0 there.
     {
       const uint32_t r0 = (uint32_t)(((uint64_t)n * Gc.me) / Gc.K), r1 = (uint32_t)(((uint64_t)n * (Gc.me + 1)) / Gc.K);
+      const uint32_t np_ = n + p;
       for (uint32_t r = r0 + tid; r < r1; r += NT) {
         uint32_t xr, zr;
         if constexpr (BIN) {
-          xr = (uint32_t)((*w64(Tg.X, r, p >> 6, rs, Tg.il) >> (p & 63)) & 1);
-          zr = (uint32_t)((*w64(Tg.Z, r, p >> 6, rs, Tg.il) >> (p & 63)) & 1);
+          const uint32_t wp = p >> 6, wn = np_ >> 6;
+          uint64_t* xpw = w64(Tg.X, r, wp, rs, Tg.il);
+          uint64_t* zpw = w64(Tg.Z, r, wp, rs, Tg.il);
+          uint64_t* xnw = w64(Tg.X, r, wn, rs, Tg.il);
+          uint64_t* znw = w64(Tg.Z, r, wn, rs, Tg.il);
+          const bool sep = wn != wp;
+          const uint64_t xa = *xpw, za = *zpw;
+          const uint64_t xb = sep ? *xnw : xa, zb = sep ? *znw : za;
+          xr = (uint32_t)((xa >> (p & 63)) & 1);
+          zr = (uint32_t)((za >> (p & 63)) & 1);
+          const uint64_t mp = 1ull << (p & 63), mn = 1ull << (np_ & 63);
+          uint64_t xn = (xb & ~mn) | (xr ? mn : 0ull), zn = (zb & ~mn) | (zr ? mn : 0ull);
+          if (!sep && (xr | zr)) {
+            xn &= ~mp;
+            zn = (zn & ~mp) | (r == q ? mp : 0ull);
+          }
+          if (xn != xb) *xnw = xn;
+          if (zn != zb) *znw = zn;
+          if (sep && (xr | zr)) {
+            *xpw = xa & ~mp;
+            *zpw = (za & ~mp) | (r == q ? mp : 0ull);
+          }
+          bytes += 32;
         } else {
           xr = *b8(Tg.X, r, p, rs, Tg.il);
           zr = *b8(Tg.Z, r, p, rs, Tg.il);
+          const uint32_t xo = *b8(Tg.X, r, np_, rs, Tg.il), zo = *b8(Tg.Z, r, np_, rs, Tg.il);
+          const uint32_t xn = (e * xr) % D, zn = (e * zr) % D;
+          if (xn != xo) *b8(Tg.X, r, np_, rs, Tg.il) = (uint8_t)xn;
+          if (zn != zo) *b8(Tg.Z, r, np_, rs, Tg.il) = (uint8_t)zn;
+          if (xr | zr) {
+            *b8(Tg.X, r, p, rs, Tg.il) = 0;
+            *b8(Tg.Z, r, p, rs, Tg.il) = (uint8_t)(r == q);
+          }
+          bytes += 6;
         }
         stg[r] = (uint8_t)(xr | (zr << 4));
       }
@@ -238,7 +287,6 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
           stg[n + s] = phase_vec(Tg, s)[p];
         }
       }
-      if (own_p) bytes += BIN ? 16ull * n : 2ull * n;
     }
     if (tid == 0) {
       if constexpr (BIN) {
@@ -311,11 +359,8 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     SDIM_TADD(P, 4, tU);
     SDIM_TMARK(tC);
     const uint32_t nT = min((uint32_t)M[1], S.tr_cap), nF = min((uint32_t)M[2], S.fl_cap);
-    uint32_t e = 1, xz = 0, pinc = 0;
+    uint32_t xz = 0, pinc = 0;
     if constexpr (!BIN) {
-      const uint32_t xv = S.xp[q];
-      #pragma unroll
-      for (uint32_t c = 1; c < D; c++) if ((xv * c) % D == 1) e = c;
       const uint32_t xz_old = (uint32_t)M[3] % D;
       xz = ((e * e) % D) * xz_old % D;
       pinc = xz_old * ((e * (e - 1) / 2) % D) % D;
@@ -327,51 +372,10 @@ SDIM_DEV bool gang_measure(const Tab& Tg, const Tab& V, uint32_t w0, uint32_t q,
     SDIM_TADD(P, 5, tC);
     SDIM_TMARK(tD);
 
-    // ---- D: own parts of columns p and n+p; per-shot phases over own words with f != 0.
-    // Column n+p is rewritten at every row (the old destabiliser's support is not known);
-    // column p only at the touched rows: elsewhere it is the pivot column's zero already.
-    if (own_g || (own_p && BIN && (p >> 6) == ((n + p) >> 6))) {
-      for (uint32_t r = tid; r < n; r += NT) {
-        if constexpr (BIN) {
-          const uint64_t xv = S.xp[r], zv = S.zp[r];
-          const uint32_t g = n + p;
-          if ((p >> 6) == (g >> 6)) {        // p and n+p share a word: both bits at every row
-            uint64_t* xw = w64(V.X, r, (p >> 6) - w0, rs, V.il);
-            uint64_t* zw = w64(V.Z, r, (p >> 6) - w0, rs, V.il);
-            uint64_t x = *xw & ~(1ull << (p & 63));
-            uint64_t z = (*zw & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-            x = (x & ~(1ull << (g & 63))) | (xv << (g & 63));
-            z = (z & ~(1ull << (g & 63))) | (zv << (g & 63));
-            *xw = x; *zw = z;
-          } else {
-            uint64_t* xw = w64(V.X, r, (g >> 6) - w0, rs, V.il);
-            uint64_t* zw = w64(V.Z, r, (g >> 6) - w0, rs, V.il);
-            *xw = (*xw & ~(1ull << (g & 63))) | (xv << (g & 63));
-            *zw = (*zw & ~(1ull << (g & 63))) | (zv << (g & 63));
-          }
-          bytes += 16;
-        } else {
-          *b8(V.X, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.xp[r]) % D);
-          *b8(V.Z, r, n + p - c_lo, rs, V.il) = (uint8_t)((e * S.zp[r]) % D);
-          bytes += 2;
-        }
-      }
-    }
-    if (own_p && !(BIN && (p >> 6) == ((n + p) >> 6))) {
-      for (uint32_t k = tid; k < nT; k += NT) {
-        const uint32_t r = S.TR[k];
-        if constexpr (BIN) {
-          uint64_t* xw = w64(V.X, r, (p >> 6) - w0, rs, V.il);
-          uint64_t* zw = w64(V.Z, r, (p >> 6) - w0, rs, V.il);
-          *xw &= ~(1ull << (p & 63));
-          *zw = (*zw & ~(1ull << (p & 63))) | ((uint64_t)(r == q) << (p & 63));
-          bytes += 16;
-        } else {
-          *b8(V.X, r, p - c_lo, rs, V.il) = 0;
-          *b8(V.Z, r, p - c_lo, rs, V.il) = (uint8_t)(r == q);
-          bytes += 2;
-        }
-      }
+    // ---- D: per-shot phases over own words with f != 0.
+    if constexpr (D != 2) {
+      rowsum_shared_terms<D, NT>(S, nF, xz);
+      __syncthreads();
     }
     rowsum_phases<D, NT>(V, S, nF, e, xz, pinc, bytes);
     __syncthreads();

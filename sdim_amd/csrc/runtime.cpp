@@ -194,6 +194,7 @@ struct sdim_ctx {
   int64_t snapshots = 0;      // tableau snapshots: 1 at every measurement event, 2 after every instruction
   int64_t sweep_rows = 1;     // HBM-resident items: rows in flight per thread in the rowsum sweep (1, 2, 4)
   int64_t seg_batches = 2;    // HBM-resident items: gate batches whose ops are loaded together (1, 2)
+  int64_t interleave = 3;     // one-workgroup batched items: X/Z rows in blocks of 2^interleave (0..3)
   int64_t serial_max = 4;     // runs of >= 2 layers of at most this many gates become one serial layer (0: off)
   hipEvent_t gang_done = nullptr; // completion of the last gang launch on any stream
   std::vector<std::vector<hipStream_t>> branch_streams;  // per submission stream: extra streams for parallel launches
@@ -793,6 +794,9 @@ int sdim_set_option(sdim_ctx* c, const char* key, int64_t v) {
   } else if (k == "seg_batches") {
     if (v != 1 && v != 2) return fail(SDIM_EINVAL, "seg_batches must be 1 or 2");
     c->seg_batches = v;
+  } else if (k == "interleave") {
+    if (v < 0 || v > 3) return fail(SDIM_EINVAL, "interleave must be in [0, 3]");
+    c->interleave = v;
   } else if (k == "serial_max") {
     if (v < 0 || v > 65535) return fail(SDIM_EINVAL, "serial_max must be in [0, 65535]");
     c->serial_max = v;
@@ -830,6 +834,7 @@ int sdim_get_option(sdim_ctx* c, const char* key, int64_t* v) {
   else if (k == "sweep_rows") *v = c->sweep_rows;
   else if (k == "seg_batches") *v = c->seg_batches;
   else if (k == "serial_max") *v = c->serial_max;
+  else if (k == "interleave") *v = c->interleave;
   else return fail(SDIM_EINVAL, "unknown option " + k);
   return 0;
 }
@@ -1007,7 +1012,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     const bool given = j.init[0] && j.init[1] && j.init[2] && j.init[3] && j.init[4] && j.init[5];
     // 8-row interleave for tableaux swept by one workgroup; gang members and the plain path
     // apply gates straight from HBM and keep rows contiguous
-    const uint32_t il = (mode == SDIM_MODE_BATCHED && !gang_cb) ? 3u : 0u;
+    const uint32_t il = (mode == SDIM_MODE_BATCHED && !gang_cb) ? (uint32_t)c->interleave : 0u;
     if (given) pack_tableau(jp, S, il, j, init_buf);
     for (int64_t s0 = 0; s0 < j.shots; s0 += S) {
       ItemDesc it;

@@ -91,7 +91,7 @@ struct ItemDesc {
   uint32_t rs;         // row stride in bytes
   uint32_t flags;
   uint32_t n_meas;
-  uint32_t il;         // X/Z row interleave: 3 (8-row blocks) or 0 (row-major)
+  uint32_t il;         // X/Z row interleave: blocks of 2^il rows (0: row-major)
   uint32_t id;         // index of the item in its submission (reported with kernel errors)
   uint64_t snap_off;   // byte offset of the item's tableau snapshots (ITEM_SNAP_MEAS, OP_SNAP), each
                        // snapshot_stride() bytes in the item's own layout

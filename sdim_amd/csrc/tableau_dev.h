@@ -342,86 +342,17 @@ SDIM_DEV bool two_rows(const DevOp& o) {
 }
 
 // ------------------------------------------------------------------ gate segment
-// ops[k] is the OP_SEG header.  Returns the index after the segment.
-template <int D, int NT, int OPB = 2>
-SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const Scr& S,
-                              const ItemDesc& it, const KParams& P, unsigned long long& bytes,
-                              unsigned long long& lbytes) {
+// The layers of one gate segment over the column words [0, NWd) of view V (the whole
+// tableau, or one column tile staged in LDS); ops[kl] is the first layer header.  Returns
+// the index after the last layer; leaves the segment's phase increments in DW.  Called
+// once per view origin so that the accesses of each call compile to one address space.
+template <int D, int NT, int OPB>
+SDIM_DEV uint32_t seg_layers(const DevOp* ops, uint32_t kl, uint32_t nl, const Tab& V, uint32_t NWd,
+                             const Scr& S, uint32_t lw, unsigned long long& gb) {
   constexpr bool BIN = (D == 2);
   constexpr int GB = 4;     // gates per load batch
   constexpr int OB = OPB;   // batches whose ops are loaded together
-  SDIM_TMARK(tSeg);
-  const DevOp hdr = ops[k];
-  const uint32_t nl = hdr.q, nn = hdr.t;
-  const uint32_t n = T.n, rs = T.rs, Sh = T.S;
   const uint32_t tid = threadIdx.x;
-  const uint32_t ndesc = k + 1;
-  k += 1 + nn;
-  unsigned long long gb = 0;                   // bytes of the gate updates
-
-  // noise draws of the segment: mask[j][S/32] bits + ab[j][S] bytes
-  const uint32_t mw = (Sh + 31) / 32;
-  const uint32_t lw = T.rs_full;                // width of the LDS per-column arrays
-  uint32_t* nmask = (uint32_t*)(S.noise + nn * 2 * lw);
-  uint8_t* nab = (uint8_t*)(nmask + nn * mw);
-  if (nn) {
-    for (uint32_t i = tid; i < nn * mw; i += NT) nmask[i] = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < nn * Sh; i += NT) {
-      const uint32_t j = i / Sh, s = i - j * Sh;
-      const DevOp nd = ops[ndesc + j];
-      const U64x4 o = draw(it.seed, it.stream, nd.src, it.shot0 + s);
-      const uint32_t ab = noise_ab(o, D, op_a(nd), P.keep53[it.keep_off + nd.t]);
-      nab[i] = (uint8_t)(ab & 0xFF);
-      if (ab) atomicOr(&nmask[j * mw + (s >> 5)], 1u << (s & 31));
-    }
-    __syncthreads();
-  }
-
-  // Column tiles: an HBM-resident tableau is staged through LDS, TW bytes of every
-  // row at a time (gates act on each generator column independently, so the tiles
-  // need no coordination); an LDS-resident tableau is a single tile.
-  uint32_t TW = rs;
-  uint8_t* tile = nullptr;
-  // phases cached in LDS are parked at their global place while the tiles run (a segment
-  // without noise ops touches them only in its final phase pass); the tiles then also
-  // cover their LDS
-  const bool park = S.tiling && (it.flags & ITEM_PH_LDS) && nn == 0;
-  uint8_t* ph_home = P.arena + it.tab_off + 2ull * rows_padded(n) * rs;
-  const uint32_t ph_bytes = (BIN ? 2u : 1u) * Sh * rs;
-  if (park) {
-    copy16<NT>(ph_home, S.gap, ph_bytes);
-    __syncthreads();
-  }
-  if (S.tiling) {
-    tile = park ? S.gap : S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
-    const int64_t left = (int64_t)(S.lds_end - tile);
-    const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
-    uint64_t tw = room / (2ull * rows_padded(n));
-    tw &= ~15ull;
-    TW = tw >= 16 ? (uint32_t)(tw < rs ? tw : rs) : 0;
-  }
-  const uint32_t CPWb = BIN ? 8 : 4;            // bytes per column word
-  for (uint32_t c0 = 0; c0 < rs; c0 += (TW ? TW : rs)) {
-    Tab V = T;
-    uint32_t vw = rs;                            // bytes of each row in this view
-    if (S.tiling && TW) {
-      vw = (rs - c0 < TW) ? rs - c0 : TW;
-      V.X = tile; V.Z = tile + (size_t)rows_padded(n) * vw; V.rs = vw; V.wofs = c0 / CPWb;
-      // load the tile: 128-byte pieces (8 rows x 16 bytes) of the X and Z blocks (tiling is
-      // used for 8-row interleaved tableaux only)
-      const uint32_t cpr = vw / 16, nrb = 2 * rows_padded(n) / 8;
-      for (uint32_t i = tid; i < nrb * cpr * 8; i += NT) {
-        const uint32_t ch = i >> 3, k = i & 7;
-        const uint32_t rb = ch / cpr, cc = ch - rb * cpr;
-        const uint8_t* src = T.X + ((size_t)rb * rs + c0 + 16 * cc) * 8 + 16 * k;
-        *(uint4*)(tile + ((size_t)rb * vw + 16 * cc) * 8 + 16 * k) = *(const uint4*)src;
-        bytes += 16;
-      }
-      __syncthreads();
-    }
-    const uint32_t NWd = (S.tiling && TW) ? vw / CPWb : T.nw;
-    uint32_t kl = k;
   // word ownership: (word, slot) pairs when a row has at most NT words (the
   // slot partial deltas are reduced at the end); otherwise each thread owns
   // several whole words and adds each (layer, word) delta into DW directly.
@@ -594,15 +525,104 @@ SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const 
     }
   }
   __syncthreads();
+  return kl;
+}
+
+// ops[k] is the OP_SEG header.  Returns the index after the segment.
+template <int D, int NT, int OPB = 2>
+SDIM_DEV uint32_t run_segment(const DevOp* ops, uint32_t k, const Tab& T, const Scr& S,
+                              const ItemDesc& it, const KParams& P, unsigned long long& bytes,
+                              unsigned long long& lbytes) {
+  constexpr bool BIN = (D == 2);
+  constexpr int GB = 4;     // gates per load batch
+  constexpr int OB = OPB;   // batches whose ops are loaded together
+  SDIM_TMARK(tSeg);
+  const DevOp hdr = ops[k];
+  const uint32_t nl = hdr.q, nn = hdr.t;
+  const uint32_t n = T.n, rs = T.rs, Sh = T.S;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ndesc = k + 1;
+  k += 1 + nn;
+  unsigned long long gb = 0;                   // bytes of the gate updates
+
+  // noise draws of the segment: mask[j][S/32] bits + ab[j][S] bytes
+  const uint32_t mw = (Sh + 31) / 32;
+  const uint32_t lw = T.rs_full;                // width of the LDS per-column arrays
+  uint32_t* nmask = (uint32_t*)(S.noise + nn * 2 * lw);
+  uint8_t* nab = (uint8_t*)(nmask + nn * mw);
+  if (nn) {
+    for (uint32_t i = tid; i < nn * mw; i += NT) nmask[i] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < nn * Sh; i += NT) {
+      const uint32_t j = i / Sh, s = i - j * Sh;
+      const DevOp nd = ops[ndesc + j];
+      const U64x4 o = draw(it.seed, it.stream, nd.src, it.shot0 + s);
+      const uint32_t ab = noise_ab(o, D, op_a(nd), P.keep53[it.keep_off + nd.t]);
+      nab[i] = (uint8_t)(ab & 0xFF);
+      if (ab) atomicOr(&nmask[j * mw + (s >> 5)], 1u << (s & 31));
+    }
+    __syncthreads();
+  }
+
+  // Column tiles: an HBM-resident tableau is staged through LDS, TW bytes of every
+  // row at a time (gates act on each generator column independently, so the tiles
+  // need no coordination); an LDS-resident tableau is a single tile.
+  uint32_t TW = rs;
+  uint8_t* tile = nullptr;
+  // phases cached in LDS are parked at their global place while the tiles run (a segment
+  // without noise ops touches them only in its final phase pass); the tiles then also
+  // cover their LDS
+  const bool park = S.tiling && (it.flags & ITEM_PH_LDS) && nn == 0;
+  uint8_t* ph_home = P.arena + it.tab_off + 2ull * rows_padded(n) * rs;
+  const uint32_t ph_bytes = (BIN ? 2u : 1u) * Sh * rs;
+  if (park) {
+    copy16<NT>(ph_home, S.gap, ph_bytes);
+    __syncthreads();
+  }
+  if (S.tiling) {
+    tile = park ? S.gap : S.noise + ((noise_bytes(nn, lw, Sh) + 15u) & ~15u);
+    const int64_t left = (int64_t)(S.lds_end - tile);
+    const uint64_t room = left > 0 ? (uint64_t)left : 0ull;
+    uint64_t tw = room / (2ull * rows_padded(n));
+    tw &= ~15ull;
+    TW = tw >= 16 ? (uint32_t)(tw < rs ? tw : rs) : 0;
+  }
+  const uint32_t CPWb = BIN ? 8 : 4;            // bytes per column word
+  for (uint32_t c0 = 0; c0 < rs; c0 += (TW ? TW : rs)) {
+    uint32_t vw = rs;                            // bytes of each row in this view
+    if (S.tiling && TW) {
+      vw = (rs - c0 < TW) ? rs - c0 : TW;
+      // load the tile: pieces of 2^il rows x 16 bytes (one block of the interleave) of the X
+      // and Z blocks; the tile keeps the interleave with row stride vw
+      const uint32_t il = T.il;
+      const uint32_t cpr = vw / 16, nrb = (2 * rows_padded(n)) >> il;
+      for (uint32_t i = tid; i < (nrb * cpr) << il; i += NT) {
+        const uint32_t ch = i >> il, k = i & ((1u << il) - 1u);
+        const uint32_t rb = ch / cpr, cc = ch - rb * cpr;
+        const uint8_t* src = T.X + (((size_t)rb * rs + c0 + 16 * cc) << il) + 16 * k;
+        *(uint4*)(tile + (((size_t)rb * vw + 16 * cc) << il) + 16 * k) = *(const uint4*)src;
+        bytes += 16;
+      }
+      __syncthreads();
+    }
+    uint32_t kl;
+    if (S.tiling && TW) {
+      Tab Vt = T;                                // a view of LDS only
+      Vt.X = tile; Vt.Z = tile + (size_t)rows_padded(n) * vw; Vt.rs = vw; Vt.wofs = c0 / CPWb;
+      kl = seg_layers<D, NT, OPB>(ops, k, nl, Vt, vw / CPWb, S, lw, gb);
+    } else {
+      kl = seg_layers<D, NT, OPB>(ops, k, nl, T, T.nw, S, lw, gb);
+    }
 
     if (S.tiling && TW) {
       // write the tile back
-      const uint32_t cpr = vw / 16, nrb = 2 * rows_padded(n) / 8;
-      for (uint32_t i = tid; i < nrb * cpr * 8; i += NT) {
-        const uint32_t ch = i >> 3, k = i & 7;
+      const uint32_t il = T.il;
+      const uint32_t cpr = vw / 16, nrb = (2 * rows_padded(n)) >> il;
+      for (uint32_t i = tid; i < (nrb * cpr) << il; i += NT) {
+        const uint32_t ch = i >> il, k = i & ((1u << il) - 1u);
         const uint32_t rb = ch / cpr, cc = ch - rb * cpr;
-        uint8_t* dst = T.X + ((size_t)rb * rs + c0 + 16 * cc) * 8 + 16 * k;
-        *(uint4*)dst = *(const uint4*)(tile + ((size_t)rb * vw + 16 * cc) * 8 + 16 * k);
+        uint8_t* dst = T.X + (((size_t)rb * rs + c0 + 16 * cc) << il) + 16 * k;
+        *(uint4*)dst = *(const uint4*)(tile + (((size_t)rb * vw + 16 * cc) << il) + 16 * k);
         bytes += 16;
       }
       __syncthreads();
@@ -850,7 +870,8 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
 }
 
 // Step D (phases): ph_i += f_i ph_p' + f_i <z_i, x_p> + xz f_i (f_i - 1) / 2 for every shot and
-// every word of the groups in FL (ph_p' = exponentiated pivot phase of the shot).
+// every word of the groups in FL (ph_p' = exponentiated pivot phase of the shot).  For odd d
+// the last two terms come from DOT as rowsum_shared_terms left them.
 template <int D, int NT>
 SDIM_DEV void rowsum_phases(const Tab& V, const Scr& S, uint32_t nF, uint32_t e, uint32_t xz, uint32_t pinc,
                             unsigned long long& bytes) {
@@ -878,20 +899,33 @@ SDIM_DEV void rowsum_phases(const Tab& V, const Scr& S, uint32_t nF, uint32_t e,
       } else {
         const uint32_t f = ((const uint32_t*)S.F)[w];
         if (!f) continue;
-        const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
-        uint32_t c = 0;
-        #pragma unroll
-        for (int kb = 0; kb < 4; kb++) {
-          const uint32_t fb = byte_of(f, kb);
-          const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
-          c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
-        }
         const uint32_t php = (e * S.shot[s] + pinc) % D;
         uint32_t* ph = (uint32_t*)phase_vec(V, s) + w;
-        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), c);
+        *ph = addm<D>(addm<D>(*ph, mult<D>(S.mtab[php], f)), S.DOT[2 * w]);
         bytes += 8;
       }
     }
+}
+
+// Odd d, before step D: the shot-independent part of each listed word's increment,
+// f <z_i, x_p> + xz f (f - 1) / 2 (bytes mod d), replaces the word's first dot word.
+template <int D, int NT>
+SDIM_DEV void rowsum_shared_terms(const Scr& S, uint32_t nF, uint32_t xz) {
+  constexpr uint32_t WPG = 4;
+  for (uint32_t i = threadIdx.x; i < nF * WPG; i += NT) {
+    const uint32_t w = S.FL[i / WPG] * WPG + i % WPG;
+    const uint32_t f = ((const uint32_t*)S.F)[w];
+    if (!f) continue;
+    const uint32_t de = S.DOT[2 * w], dO = S.DOT[2 * w + 1];
+    uint32_t c = 0;
+    #pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      const uint32_t fb = byte_of(f, kb);
+      const uint32_t dot = ((kb & 1) ? (dO >> (16 * (kb >> 1))) : (de >> (16 * (kb >> 1)))) & 0xFFFFu;
+      c |= ((fb * (dot % D) + xz * ((fb * (fb - 1) / 2) % D)) % D) << (8 * kb);
+    }
+    S.DOT[2 * w] = c;
+  }
 }
 
 // ------------------------------------------------------------------ snapshots
@@ -1041,6 +1075,10 @@ SDIM_DEV void measure(const Tab& T, uint32_t q, uint32_t slot, uint32_t src, boo
     SDIM_TMARK(tD);
 
     // ---- D: per-shot phases over words with f != 0
+    if constexpr (!BIN) {
+      rowsum_shared_terms<D, NT>(S, nF, xz);
+      __syncthreads();
+    }
     rowsum_phases<D, NT>(T, S, nF, e, xz, pinc, bytes);
     __syncthreads();                                                      // B4
     SDIM_TADD(P, 3, tD);
@@ -1262,7 +1300,7 @@ SDIM_DEV void run_item_at(const ItemDesc& it, const KParams& P, uint8_t* lds) {
   S.item = it.id;
   if (P.list_cap) { S.tr_cap = min(S.tr_cap, P.list_cap); S.fl_cap = min(S.fl_cap, P.list_cap); }
   S.lds_end = lds + P.lds_bytes;
-  S.tiling = !res && it.il == 3;
+  S.tiling = !res;
   Tab T;
   T.X = tab; T.Z = tab + (size_t)rows_padded(n) * rs; T.Ph = phl ? S.gap : tab + XZ_bytes; T.n = n; T.rs = rs; T.S = Sh;
   T.wofs = 0; T.rs_full = rs; T.nw = rs / (D == 2 ? 8 : 4); T.il = it.il;
