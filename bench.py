@@ -469,7 +469,12 @@ def main():
             return None
         c = classes[cls]
         ach = c["bytes"] / (c["ms"] * 1e-3) / 1e9
-        traffic = profiled_traffic("batch_kernel_uneven_batch") if cls == "hbm" and args.workload == "uneven" else None
+        traffic = None
+        if args.workload == "uneven":
+            # HBM bytes (FETCH_SIZE + WRITE_SIZE) of the HBM class, LDS transfers of the
+            # LDS-resident classes, both from the rocprofv3 counter runs in the profile
+            traffic = (profiled_traffic("batch_kernel_uneven_batch") if cls == "hbm"
+                       else profiled_value("batch_kernel_uneven_batch", "lds_traffic_bytes_per_launch"))
         return {"bound": "hbm" if cls == "hbm" else "lds", "achieved": round(ach, 1), "peak": peak,
                 "unit": "GB/s", "frac": round(ach / peak, 4), "traffic": traffic,
                 "traffic_source": "profiles/batch_kernel_uneven_batch.md" if traffic else None,

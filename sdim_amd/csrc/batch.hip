@@ -12,6 +12,9 @@
 #ifndef SDIM_WAVES_PER_EU
 #define SDIM_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
+#ifndef SDIM_BATCH_WAVES
+#define SDIM_BATCH_WAVES 4
+#endif
 
 namespace sdim {
 
@@ -22,7 +25,7 @@ namespace sdim {
 // gate batches per op load (the HBM-resident classes pick them per launch, the LDS-resident
 // classes use RB = 1, OPB = 2).
 template <int NT, bool RES, int RB, int OPB>
-__global__ __launch_bounds__(NT) SDIM_WAVES_PER_EU(4) void batch_kernel(KParams P) {
+__global__ __launch_bounds__(NT) SDIM_WAVES_PER_EU(SDIM_BATCH_WAVES) void batch_kernel(KParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t s_item;
   for (;;) {
