@@ -231,7 +231,7 @@ def serving(eng, n_requests, rate, budget_gb, seed=5):
             "burst": {"requests": len(burst), "seconds": round(burst_s, 4),
                       "requests_per_s": round(len(burst) / burst_s, 2),
                       "gate_shots_per_s": burst_gs / burst_s},
-            "gate_shot_share": {"n<256": share(0, 256), f"256<=n<{gmin} (one workgroup)": share(256, gmin),
+            "gate_shot_share": {"n<256": share(0, 256), f"256<=n<{gmin} (one-workgroup items)": share(256, gmin),
                                 f"n>={gmin} (gang)": share(gmin, 1 << 40)},
             "requests_by_band": [sum(1 for r in reqs if r.n < 256), sum(1 for r in reqs if 256 <= r.n < gmin),
                                  sum(1 for r in reqs if r.n >= gmin)],

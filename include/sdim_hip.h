@@ -77,8 +77,9 @@ const char* sdim_last_error(void);
  * Tuning / engine options (no reference counterpart).  Keys:
  *   "threads"         workgroup size of the batch kernel: 0 (per launch class), 256, 512 or 1024
  *   "lds_bytes"       upper bound of the LDS budget per workgroup (<= 163584)
- *   "shots_per_item"  shots that share one X/Z block per work item (0 = automatic, 1 = a
- *                     separate tableau per shot)
+ *   "shots_per_item"  shots that share one X/Z block per work item (0 = automatic: enough
+ *                     items to fill the device twice, but at least min(64, n/16) shots per
+ *                     item; 1 = a separate tableau per shot)
  *   "grid"            persistent grid size (0 = automatic)
  *   "allow_lds"       0 forces every tableau to stay in HBM
  *   "hbm_threads"     workgroup size of the launch class for small HBM-resident items (64..1024,

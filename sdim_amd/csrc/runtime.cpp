@@ -913,7 +913,13 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
 
     const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = row_stride(n, d);
     const bool has_noise = keep.size() > keep0;
-    uint32_t S = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : auto_S);
+    // every item repeats the circuit's X/Z evolution, so the automatic split keeps at least
+    // min(64, n / 16) shots per item: below that a large tableau's X/Z work swamps the phase
+    // work it parallelises (n = 1012, d = 11, 2436 shots alone: 256 ms at one shot per item,
+    // 18 ms at 64)
+    const uint32_t floor_S = std::min<uint32_t>(64u, std::max<uint32_t>(1u, (uint32_t)j.n / 16u));
+    uint32_t S = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? (int64_t)c->shots_per_item
+                                                                        : (int64_t)std::max(auto_S, floor_S));
     if (opts.one_item_per_job) {
       if (j.shots > 4096) return fail(SDIM_EINVAL, "at most 4096 shots per job on this path (one item per job)");
       S = (uint32_t)j.shots;
@@ -978,19 +984,20 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       if (scratch_bytes(n, rs, S, 256) > 163584) return fail(SDIM_ENOMEM, "circuit too large for the plain path");
     }
     const bool resident = cls >= 0 && kClasses[cls].resident;
-    jp.S_item = S;
-    jp.resident = resident;
-    // one-workgroup HBM items keep the shots' phase vectors in LDS when they take at most
-    // a quarter of the class budget (the rest stays for scratch, noise tables and tiles)
     bool ph_lds = false;
-    if (mode == SDIM_MODE_BATCHED && !resident && !gang_cb) {
-      const uint64_t ph = tableau_bytes(n, d, S) - 2ull * rows_padded(n) * rs;
-      const uint64_t budget = std::min<uint32_t>(kClasses[cls].lds, cap);
-      ph_lds = ph <= budget / 4 &&
-               ((ph + 15) & ~15ull) + scratch_bytes(n, rs, S, nt_of(c, kClasses[cls])) + (has_noise ? noise_bytes(1, rs, S) : 0) <= budget;
-    }
-    // noise ops per segment that fit next to the tableau / scratch
-    if (mode == SDIM_MODE_BATCHED) {
+    uint32_t nn_max = 0;
+    for (;;) {
+      // one-workgroup HBM items keep the shots' phase vectors in LDS when they take at most
+      // a quarter of the class budget (the rest stays for scratch, noise tables and tiles)
+      ph_lds = false;
+      if (mode == SDIM_MODE_BATCHED && !resident && !gang_cb) {
+        const uint64_t ph = tableau_bytes(n, d, S) - 2ull * rows_padded(n) * rs;
+        const uint64_t budget = std::min<uint32_t>(kClasses[cls].lds, cap);
+        ph_lds = ph <= budget / 4 &&
+                 ((ph + 15) & ~15ull) + scratch_bytes(n, rs, S, nt_of(c, kClasses[cls])) + (has_noise ? noise_bytes(1, rs, S) : 0) <= budget;
+      }
+      if (mode != SDIM_MODE_BATCHED) break;
+      // noise ops per segment that fit next to the tableau / scratch
       const LaunchClass& L = kClasses[cls];
       const uint32_t w = gang_cb ? gang_cb : rs;     // width of the per-column LDS arrays
       const uint32_t nt = gang_cb ? (uint32_t)L.nt : nt_of(c, L);
@@ -999,9 +1006,16 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       const uint64_t budget = std::min<uint32_t>(L.lds, cap);
       // HBM-resident one-workgroup items keep half of the remaining LDS for the column tiles of gate segments
       const uint64_t room = budget > base ? ((resident || gang_cb) ? budget - base : (budget - base) / 2) : 0;
-      uint32_t nn_max = 0;
+      nn_max = 0;
       while (nn_max < 65535 && noise_bytes(nn_max + 1, w, S) <= room) nn_max++;
-      if (has_noise && nn_max == 0) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
+      if (!has_noise || nn_max > 0) break;
+      // an automatic shot count halves until a noise table fits (the noise tables grow with it)
+      if (S == 1 || opts.one_item_per_job || c->shots_per_item) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
+      S /= 2;
+    }
+    jp.S_item = S;
+    jp.resident = resident;
+    if (mode == SDIM_MODE_BATCHED) {
       jp.ops_off = (uint32_t)ops.size();
       build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, (uint32_t)c->serial_max, ops);
       jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
