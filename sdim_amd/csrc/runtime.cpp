@@ -24,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -890,10 +891,25 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
   std::vector<int> cls_of;
   auto& inits = st.inits;
 
-  uint64_t rec_total = 0;
-  for (int32_t ji = 0; ji < n_jobs; ji++) {
+  // Lowering, class choice and layering of each job depend on that job alone: large
+  // submissions run them on worker threads; the results are merged in job order.
+  struct JobWork {
+    int rc = 0;
+    std::string err;
+    std::vector<uint64_t> keep;
+    std::vector<DevOp> ops;
+    std::vector<uint8_t> init_buf;
+    LaunchClass gL{false, 0, 0, 1, 0, 0};
+    int cls = -1;
+    uint32_t S = 1, rs = 0, il = 0, gang_cb = 0;
+    bool resident = false, ph_lds = false, given = false;
+  };
+  std::vector<JobWork> work((size_t)std::max(n_jobs, 0));
+  auto plan_job = [&](int32_t ji) -> int {
     const sdim_job& j = jobs[ji];
     JobPlan& jp = T->jobs[(size_t)ji];
+    JobWork& w = work[(size_t)ji];
+
     if (j.n < 1) return fail(SDIM_EINVAL, "Number of qudits must be greater than 0");
     if (j.d < 2) return fail(SDIM_EINVAL, "Dimension must be greater than 1");
     if (!supported_dim(j.d)) return fail(SDIM_EINVAL, "dimension " + std::to_string(j.d) + " is not supported (prime d in {2,3,5,7,11})");
@@ -903,16 +919,12 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     jp.n = j.n;
     jp.d = j.d;
     jp.shots = j.shots;
-    jp.keep_off = keep.size();
-    const size_t keep0 = keep.size();
-    int rc = expand(j, jp, keep, c->snapshots == 2);
+    int rc = expand(j, jp, w.keep, c->snapshots == 2);
     if (rc) return rc;
     jp.n_snap = c->snapshots == 1 ? jp.n_meas : c->snapshots == 2 ? j.n_ops : 0;
-    // keep indices in prims are global; make them circuit-relative
-    for (auto& p : jp.prims) if (p.kind == OP_NOISE) p.aux -= (uint32_t)keep0;
 
     const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = row_stride(n, d);
-    const bool has_noise = keep.size() > keep0;
+    const bool has_noise = !w.keep.empty();
     // every item repeats the circuit's X/Z evolution, so the automatic split keeps at least
     // min(64, n / 16) shots per item: below that a large tableau's X/Z work swamps the phase
     // work it parallelises (n = 1012, d = 11, 2436 shots alone: 256 ms at one shot per item,
@@ -947,18 +959,11 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
         if (gneed(Sg) <= lds) {
           S = Sg;
           gang_cb = cb;
-          for (int k = 0; k < (int)T->classes.size() && cls < 0; k++)
-            if (T->classes[k].gang == K && T->classes[k].nt == gnt && T->classes[k].d == (int)d) cls = k;
-          if (cls < 0) {
-            if (T->classes.size() >= SDIM_MAX_CLASSES) return fail(SDIM_EINVAL, "too many launch classes");
-            LaunchClass L{false, gnt, lds, 1, K, (int)d};
-            T->classes.push_back(L);
-            cls = (int)T->classes.size() - 1;
-          }
+          w.gL = LaunchClass{false, gnt, lds, 1, K, (int)d};   // found or added when the jobs are merged
         }
       }
     }
-    if (mode == SDIM_MODE_BATCHED && cls < 0) {
+    if (mode == SDIM_MODE_BATCHED && cls < 0 && !gang_cb) {
       auto need = [&](int k, uint32_t s) -> uint64_t {
         const LaunchClass& L = kClasses[k];
         return (L.resident ? ((tableau_bytes(n, d, s) + 15) & ~15ull) : 0) + scratch_bytes(n, rs, s, nt_of(c, L)) +
@@ -998,16 +1003,16 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       }
       if (mode != SDIM_MODE_BATCHED) break;
       // noise ops per segment that fit next to the tableau / scratch
-      const LaunchClass& L = kClasses[cls];
-      const uint32_t w = gang_cb ? gang_cb : rs;     // width of the per-column LDS arrays
+      const LaunchClass& L = gang_cb ? w.gL : kClasses[cls];
+      const uint32_t cw = gang_cb ? gang_cb : rs;    // width of the per-column LDS arrays
       const uint32_t nt = gang_cb ? (uint32_t)L.nt : nt_of(c, L);
-      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, w, S, nt) +
+      const uint64_t base = (resident ? ((tableau_bytes(n, d, S) + 15) & ~15ull) : 0) + scratch_bytes(n, cw, S, nt) +
                             (ph_lds ? ((tableau_bytes(n, d, S) - 2ull * rows_padded(n) * rs + 15) & ~15ull) : 0);
       const uint64_t budget = std::min<uint32_t>(L.lds, cap);
       // HBM-resident one-workgroup items keep half of the remaining LDS for the column tiles of gate segments
       const uint64_t room = budget > base ? ((resident || gang_cb) ? budget - base : (budget - base) / 2) : 0;
       nn_max = 0;
-      while (nn_max < 65535 && noise_bytes(nn_max + 1, w, S) <= room) nn_max++;
+      while (nn_max < 65535 && noise_bytes(nn_max + 1, cw, S) <= room) nn_max++;
       if (!has_noise || nn_max > 0) break;
       // an automatic shot count halves until a noise table fits (the noise tables grow with it)
       if (S == 1 || opts.one_item_per_job || c->shots_per_item) return fail(SDIM_ENOMEM, "no LDS room for noise tables");
@@ -1016,18 +1021,65 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
     jp.S_item = S;
     jp.resident = resident;
     if (mode == SDIM_MODE_BATCHED) {
-      jp.ops_off = (uint32_t)ops.size();
-      build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), jp.keep_off, (uint32_t)c->serial_max, ops);
-      jp.n_ops = (uint32_t)ops.size() - jp.ops_off;
+      build_segments(jp.prims, n, std::max<uint32_t>(nn_max, 1), 0, (uint32_t)c->serial_max, w.ops);
+      jp.n_ops = (uint32_t)w.ops.size();
     }
-    // items
-    jp.first_item = (uint32_t)items.size();
-    std::vector<uint8_t> init_buf;
     const bool given = j.init[0] && j.init[1] && j.init[2] && j.init[3] && j.init[4] && j.init[5];
     // 8-row interleave for tableaux swept by one workgroup; gang members and the plain path
     // apply gates straight from HBM and keep rows contiguous
     const uint32_t il = (mode == SDIM_MODE_BATCHED && !gang_cb) ? (uint32_t)c->interleave : 0u;
-    if (given) pack_tableau(jp, S, il, j, init_buf);
+    if (given) pack_tableau(jp, S, il, j, w.init_buf);
+    w.cls = cls; w.S = S; w.rs = rs; w.il = il; w.gang_cb = gang_cb;
+    w.resident = resident; w.ph_lds = ph_lds; w.given = given;
+    return 0;
+  };
+  {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int32_t nth = n_jobs >= 64 ? (int32_t)std::min<unsigned>(16u, hw) : 1;
+    if (nth <= 1) {
+      for (int32_t ji = 0; ji < n_jobs; ji++) {
+        work[(size_t)ji].rc = plan_job(ji);
+        if (work[(size_t)ji].rc) { work[(size_t)ji].err = g_err; break; }
+      }
+    } else {
+      std::atomic<int32_t> next{0};
+      std::vector<std::thread> pool;
+      for (int32_t t = 0; t < nth; t++)
+        pool.emplace_back([&]() {
+          for (int32_t ji; (ji = next.fetch_add(1)) < n_jobs;) {
+            work[(size_t)ji].rc = plan_job(ji);
+            if (work[(size_t)ji].rc) work[(size_t)ji].err = g_err;
+          }
+        });
+      for (auto& th : pool) th.join();
+    }
+  }
+  uint64_t rec_total = 0;
+  for (int32_t ji = 0; ji < n_jobs; ji++) {
+    const sdim_job& j = jobs[ji];
+    JobPlan& jp = T->jobs[(size_t)ji];
+    JobWork& w = work[(size_t)ji];
+    if (w.rc) return fail(w.rc, w.err);
+    int cls = w.cls;
+    if (w.gang_cb) {
+      for (int k = 0; k < (int)T->classes.size() && cls < 0; k++)
+        if (T->classes[k].gang == w.gL.gang && T->classes[k].nt == w.gL.nt && T->classes[k].d == w.gL.d) cls = k;
+      if (cls < 0) {
+        if (T->classes.size() >= SDIM_MAX_CLASSES) return fail(SDIM_EINVAL, "too many launch classes");
+        T->classes.push_back(w.gL);
+        cls = (int)T->classes.size() - 1;
+      }
+    }
+    jp.keep_off = keep.size();
+    keep.insert(keep.end(), w.keep.begin(), w.keep.end());
+    if (mode == SDIM_MODE_BATCHED) {
+      jp.ops_off = (uint32_t)ops.size();
+      ops.insert(ops.end(), w.ops.begin(), w.ops.end());
+    }
+    const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d, rs = w.rs, S = w.S, il = w.il;
+    const bool resident = w.resident, ph_lds = w.ph_lds, given = w.given;
+    // items
+    jp.first_item = (uint32_t)items.size();
     for (int64_t s0 = 0; s0 < j.shots; s0 += S) {
       ItemDesc it;
       memset(&it, 0, sizeof(it));
@@ -1048,7 +1100,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       it.id = (uint32_t)items.size();
       it.rec_off = rec_total;
       rec_total = align256(rec_total + (uint64_t)jp.n_meas * it.S);
-      if (given) inits.emplace_back((uint32_t)items.size(), init_buf);
+      if (given) inits.emplace_back((uint32_t)items.size(), w.init_buf);
       items.push_back(it);
       cls_of.push_back(cls);
       // LPT cost estimate: measurement sweeps (rows x row bytes) + per-shot phase passes + gates
