@@ -32,7 +32,8 @@ def test_random_circuits_small(engine, mode, d):
 
 @pytest.mark.parametrize("opts", [dict(), dict(allow_lds=0), dict(shots_per_item=1), dict(threads=256),
                                   dict(threads=512, allow_lds=0), dict(serial_max=0),
-                                  dict(serial_max=1000, allow_lds=0)])
+                                  dict(serial_max=1000, allow_lds=0), dict(interleave=0, allow_lds=0),
+                                  dict(interleave=2, allow_lds=0, lds_bytes=32768)])
 def test_medium_circuits_engine_options(opts):
     import zlib
     from sdim_amd.engine import Engine
