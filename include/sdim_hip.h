@@ -30,6 +30,8 @@ extern "C" {
 #define SDIM_ENOMEM -3     /* device memory exhausted / over budget */
 #define SDIM_ENOTFOUND -4  /* unknown ticket or job */
 #define SDIM_EBUSY -5      /* ticket not finished (sdim_poll) */
+#define SDIM_EINTERNAL -6  /* an internal error of the library (message in sdim_last_error); host
+                              memory exhaustion inside an entry point is SDIM_ENOMEM */
 
 #define SDIM_MODE_BATCHED 0  /* persistent kernel walking device-resident gate lists */
 #define SDIM_MODE_PLAIN 1    /* one kernel launch per primitive op */

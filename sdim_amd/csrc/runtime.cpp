@@ -43,6 +43,27 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// C++ exceptions do not cross the C ABI: the entry points that allocate host memory end in this
+// handler (a function-try-block), so running out of host memory returns SDIM_ENOMEM and any other
+// exception SDIM_EINTERNAL, with the message in sdim_last_error().
+#define SDIM_CATCH                                                                                 \
+  catch (const std::bad_alloc&) {                                                                  \
+    return fail(SDIM_ENOMEM, "host memory exhausted");                                             \
+  }                                                                                                \
+  catch (const std::exception& ex_) {                                                              \
+    return fail(SDIM_EINTERNAL, std::string("internal error: ") + ex_.what());                     \
+  }
+
+// f() with an exception turned into an error code, for threads of the library's own (the planner's
+// workers, the serving loop), from which an exception must not escape either.
+template <class F>
+int no_throw(F&& f) {
+  try {
+    return f();
+  }
+  SDIM_CATCH
+}
+
 #define HIP_OK(call)                                                                  \
   do {                                                                                \
     hipError_t e_ = (call);                                                           \
@@ -581,6 +602,23 @@ void release_ticket(sdim_ctx* c, Ticket* t) {
   t->fork_join.clear();
 }
 
+// A submission's ticket until it is handed to the context: an early return (or an exception)
+// hands its device memory, events and graphs back, once the work already enqueued on its
+// streams (and its child's) has ended.
+struct TicketUndo {
+  sdim_ctx* c;
+  Ticket* t;
+  bool armed;
+  ~TicketUndo() {
+    if (!armed) return;
+    for (Ticket* u = t; u; u = u->child.get()) {
+      if (u->stream) (void)hipStreamSynchronize(u->stream);
+      for (hipStream_t b : u->branches) (void)hipStreamSynchronize(b);
+    }
+    release_ticket(c, t);
+  }
+};
+
 // releases a ticket and forgets it
 void drop_ticket(sdim_ctx* c, std::map<uint64_t, std::unique_ptr<Ticket>>::iterator it) {
   release_ticket(c, it->second.get());
@@ -682,7 +720,7 @@ extern "C" {
 
 const char* sdim_last_error(void) { return g_err.c_str(); }
 
-int sdim_ctx_create(int32_t device, sdim_ctx** out) {
+int sdim_ctx_create(int32_t device, sdim_ctx** out) try {
   if (!out) return fail(SDIM_EINVAL, "null output pointer");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
@@ -712,7 +750,7 @@ int sdim_ctx_create(int32_t device, sdim_ctx** out) {
   if (noalloc_init_device(device)) return fail(SDIM_EHIP, "could not find the allocation-free path's device scratch");
   *out = c.release();
   return 0;
-}
+} SDIM_CATCH
 
 int sdim_ctx_destroy(sdim_ctx* c) {
   if (!c) return 0;
@@ -1043,14 +1081,20 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       }
     } else {
       std::atomic<int32_t> next{0};
+      auto worker = [&]() {
+        for (int32_t ji; (ji = next.fetch_add(1)) < n_jobs;) {
+          work[(size_t)ji].rc = no_throw([&] { return plan_job(ji); });
+          if (work[(size_t)ji].rc) work[(size_t)ji].err = g_err;
+        }
+      };
       std::vector<std::thread> pool;
-      for (int32_t t = 0; t < nth; t++)
-        pool.emplace_back([&]() {
-          for (int32_t ji; (ji = next.fetch_add(1)) < n_jobs;) {
-            work[(size_t)ji].rc = plan_job(ji);
-            if (work[(size_t)ji].rc) work[(size_t)ji].err = g_err;
-          }
-        });
+      pool.reserve((size_t)nth);
+      try {
+        for (int32_t t = 1; t < nth; t++) pool.emplace_back(worker);
+      } catch (const std::system_error&) {
+        // fewer threads than asked for: the ones started and this thread share the jobs
+      }
+      worker();
       for (auto& th : pool) th.join();
     }
   }
@@ -1246,6 +1290,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   if (!c || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
   HIP_OK(hipSetDevice(c->device));
   std::unique_ptr<Ticket> T(new Ticket());
+  TicketUndo undo{c, T.get(), true};
   Staging st;
   int rc = plan_locked(c, jobs, n_jobs, mode, stream_idx, hbm_threads, hbm_per_cu, PlanOpts(), T.get(), st);
   if (rc) return rc;
@@ -1302,6 +1347,7 @@ static int submit_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int3
   *ticket = T->id;
   if (device_bytes) *device_bytes = T->arena.bytes + T->meta.bytes + T->gang.bytes;
   c->tickets[T->id] = std::move(T);
+  undo.armed = false;
   return 0;
 }
 
@@ -1351,22 +1397,22 @@ int simulate_noalloc_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, v
 
 extern "C" {
 
-int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) {
+int sdim_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode, uint64_t* ticket) try {
   if (!c) return fail(SDIM_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
   return submit_locked(c, jobs, n_jobs, mode, 0, ticket, nullptr);
-}
+} SDIM_CATCH
 
 int sdim_simulate_noalloc(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, void* out, uint64_t out_capacity,
-                          uint64_t* out_bytes) {
+                          uint64_t* out_bytes) try {
   if (!c) return fail(SDIM_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
   return simulate_noalloc_locked(c, jobs, n_jobs, out, out_capacity, out_bytes, noalloc_scratch(c->device),
                                  SDIM_NOALLOC_SCRATCH_BYTES);
-}
+} SDIM_CATCH
 
 int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const void* const* refs, int64_t shot0,
-                      uint64_t* ticket) {
+                      uint64_t* ticket) try {
   if (!c) return fail(SDIM_EINVAL, "null context");
   if ((!jobs && n_jobs) || n_jobs < 0 || !ticket) return fail(SDIM_EINVAL, "bad submit arguments");
   if (shot0 < 0) return fail(SDIM_EINVAL, "shot0 must be >= 0");
@@ -1375,6 +1421,7 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
   std::lock_guard<std::mutex> lk(c->mu);
   HIP_OK(hipSetDevice(c->device));
   std::unique_ptr<Ticket> T(new Ticket());
+  TicketUndo undo{c, T.get(), true};
   T->mode = MODE_FRAME;
   T->stream = c->stream;
   T->jobs.resize((size_t)n_jobs);
@@ -1407,15 +1454,9 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
     T->child = std::move(it->second);
     c->tickets.erase(it);
   }
-  // errors from here on: wait for the reference shots in flight and free them
-  auto bail = [&](int code) {
-    if (T->child) {
-      (void)hipStreamSynchronize(T->child->stream);
-      release_ticket(c, T->child.get());
-      T->child.reset();
-    }
-    return code;
-  };
+  // errors from here on return through bail(): the guard above waits for the reference shots in
+  // flight and frees them with the rest of the ticket
+  auto bail = [](int code) { return code; };
 #define FRAME_HIP_OK(call)                                                                      \
   do {                                                                                          \
     hipError_t e_ = (call);                                                                     \
@@ -1514,19 +1555,20 @@ int sdim_frame_submit(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, const v
   T->id = c->next_ticket++;
   *ticket = T->id;
   c->tickets[T->id] = std::move(T);
+  undo.armed = false;
   return 0;
-}
+} SDIM_CATCH
 #undef FRAME_HIP_OK
 
-int sdim_wait(sdim_ctx* c, uint64_t ticket) {
+int sdim_wait(sdim_ctx* c, uint64_t ticket) try {
   if (!c) return fail(SDIM_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
   if (!t) return fail(SDIM_ENOTFOUND, "unknown ticket");
   return finish(c, t);
-}
+} SDIM_CATCH
 
-int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) {
+int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) try {
   if (!c || !done) return fail(SDIM_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1542,7 +1584,7 @@ int sdim_poll(sdim_ctx* c, uint64_t ticket, int32_t* done) {
   }
   *done = 1;
   return finish(c, t);
-}
+} SDIM_CATCH
 
 int sdim_release(sdim_ctx* c, uint64_t ticket) {
   if (!c) return fail(SDIM_EINVAL, "null context");
@@ -1564,7 +1606,7 @@ int sdim_num_measurements(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t* co
   return 0;
 }
 
-int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int64_t capacity) {
+int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int64_t capacity) try {
   if (!c || !out) return fail(SDIM_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1622,10 +1664,10 @@ int sdim_fetch_records(sdim_ctx* c, uint64_t ticket, int32_t job, void* out, int
     }
   }
   return 0;
-}
+} SDIM_CATCH
 
 int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, int64_t* x, int64_t* z,
-                       int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) {
+                       int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) try {
   if (!c || !x || !z || !ph || !dx || !dz || !dph) return fail(SDIM_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1650,10 +1692,10 @@ int sdim_fetch_tableau(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t shot, 
     }
   }
   return fail(SDIM_ENOTFOUND, "shot not found");
-}
+} SDIM_CATCH
 
 int sdim_fetch_snapshot(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t index, int64_t shot, int64_t* x,
-                        int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) {
+                        int64_t* z, int64_t* ph, int64_t* dx, int64_t* dz, int64_t* dph) try {
   if (!c || !x || !z || !ph || !dx || !dz || !dph) return fail(SDIM_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1676,7 +1718,7 @@ int sdim_fetch_snapshot(sdim_ctx* c, uint64_t ticket, int32_t job, int64_t index
     }
   }
   return fail(SDIM_ENOTFOUND, "shot not found");
-}
+} SDIM_CATCH
 
 int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t* bytes_hbm, uint64_t* bytes_lds,
                       int64_t* n_items, int64_t* n_lds_items) {
@@ -1696,7 +1738,7 @@ int sdim_ticket_stats(sdim_ctx* c, uint64_t ticket, double* kernel_ms, uint64_t*
   return 0;
 }
 
-int sdim_ticket_info(sdim_ctx* c, uint64_t ticket, const char* key, int64_t* value) {
+int sdim_ticket_info(sdim_ctx* c, uint64_t ticket, const char* key, int64_t* value) try {
   if (!c || !key || !value) return fail(SDIM_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1717,7 +1759,7 @@ int sdim_ticket_info(sdim_ctx* c, uint64_t ticket, const char* key, int64_t* val
     return 0;
   }
   return fail(SDIM_EINVAL, "unknown ticket key " + k);
-}
+} SDIM_CATCH
 
 int sdim_ticket_counters(sdim_ctx* c, uint64_t ticket, uint64_t* out, int32_t count) {
   if (!c || !out || count < 0) return fail(SDIM_EINVAL, "bad arguments");
@@ -1748,7 +1790,7 @@ int sdim_ticket_poison(sdim_ctx* c, uint64_t ticket, int32_t value) {
   return 0;
 }
 
-int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms) {
+int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, double* elapsed_ms) try {
   if (!c || iters < 1 || which < 0 || which > 2) return fail(SDIM_EINVAL, "bad replay arguments");
   std::lock_guard<std::mutex> lk(c->mu);
   Ticket* t = find_ticket(c, ticket);
@@ -1813,7 +1855,7 @@ int sdim_replay(sdim_ctx* c, uint64_t ticket, int32_t iters, int32_t which, doub
   HIP_OK(hipEventElapsedTime(&ms, t->ev0, t->ev1));
   if (elapsed_ms) *elapsed_ms = ms;
   return 0;
-}
+} SDIM_CATCH
 
 }  // extern "C"
 
@@ -1920,6 +1962,34 @@ void serve_loop(sdim_server* s) {
       }
       if (!done) { still.push_back(b); continue; }
       const double t_done = now_s();
+      std::vector<std::pair<uint64_t, std::vector<uint8_t>>> out;
+      if (!rc) {
+        // records of every request, in the layout of sdim_fetch_records
+        rc = no_throw([&]() -> int {
+          std::lock_guard<std::mutex> lk(c->mu);
+          Ticket* t = find_ticket(c, b.ticket);
+          for (size_t k = 0; k < b.reqs.size(); k++) {
+            const JobPlan& jp = t->jobs[k];
+            std::vector<uint8_t> buf((size_t)(jp.n_meas * jp.shots) * 33);
+            for (int64_t ev = 0; ev < jp.n_meas; ev++) {
+              for (uint32_t i = 0; i < jp.n_items; i++) {
+                const ItemDesc& it = t->items[jp.first_item + i];
+                const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
+                for (uint32_t sh = 0; sh < it.S; sh++) {
+                  const int64_t shot = (int64_t)it.shot0 + sh;
+                  uint8_t* rec = buf.data() + 33 * (size_t)(ev * jp.shots + shot);
+                  const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], val = r[sh] & 0x7F;
+                  memcpy(rec, &q, 8); memcpy(rec + 8, &round, 8); memcpy(rec + 16, &shot, 8);
+                  rec[24] = (r[sh] >> 7) & 1; memcpy(rec + 25, &val, 8);
+                }
+              }
+            }
+            out.emplace_back(b.reqs[k], std::move(buf));
+          }
+          return 0;
+        });
+        if (rc) { err = g_err; out.clear(); }
+      }
       if (rc) {
         // the whole batch failed: its requests report the error instead of records
         {
@@ -1940,29 +2010,8 @@ void serve_loop(sdim_server* s) {
         s->cv.notify_all();
         continue;
       }
-      std::vector<std::pair<uint64_t, std::vector<uint8_t>>> out;
       {
         std::lock_guard<std::mutex> lk(c->mu);
-        Ticket* t = find_ticket(c, b.ticket);
-        for (size_t k = 0; k < b.reqs.size(); k++) {
-          const JobPlan& jp = t->jobs[k];
-          std::vector<uint8_t> buf((size_t)(jp.n_meas * jp.shots) * 33);
-          // same record layout as sdim_fetch_records
-          for (int64_t ev = 0; ev < jp.n_meas; ev++) {
-            for (uint32_t i = 0; i < jp.n_items; i++) {
-              const ItemDesc& it = t->items[jp.first_item + i];
-              const uint8_t* r = t->h_rec.data() + it.rec_off + (size_t)ev * it.S;
-              for (uint32_t sh = 0; sh < it.S; sh++) {
-                const int64_t shot = (int64_t)it.shot0 + sh;
-                uint8_t* rec = buf.data() + 33 * (size_t)(ev * jp.shots + shot);
-                const int64_t q = jp.meas_q[(size_t)ev], round = jp.meas_round[(size_t)ev], val = r[sh] & 0x7F;
-                memcpy(rec, &q, 8); memcpy(rec + 8, &round, 8); memcpy(rec + 16, &shot, 8);
-                rec[24] = (r[sh] >> 7) & 1; memcpy(rec + 25, &val, 8);
-              }
-            }
-          }
-          out.emplace_back(b.reqs[k], std::move(buf));
-        }
         auto it = c->tickets.find(b.ticket);
         if (it != c->tickets.end()) drop_ticket(c, it);
       }
@@ -2027,9 +2076,11 @@ void serve_loop(sdim_server* s) {
     int rc;
     {
       std::lock_guard<std::mutex> lk(c->mu);
-      rc = submit_locked(c, jobs.data(), (int32_t)jobs.size(), SDIM_MODE_BATCHED,
-                         (int)(s->inflight.size() % c->streams.size()), &ticket, &dev_bytes,
-                         c->serve_hbm_threads, c->serve_hbm_per_cu);
+      rc = no_throw([&] {
+        return submit_locked(c, jobs.data(), (int32_t)jobs.size(), SDIM_MODE_BATCHED,
+                             (int)(s->inflight.size() % c->streams.size()), &ticket, &dev_bytes,
+                             c->serve_hbm_threads, c->serve_hbm_per_cu);
+      });
     }
     const double t_launch = now_s();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -2054,19 +2105,19 @@ void serve_loop(sdim_server* s) {
 extern "C" {
 
 int sdim_server_create(sdim_ctx* c, uint64_t hbm_budget_bytes, int32_t max_inflight, int32_t max_batch,
-                       sdim_server** out) {
+                       sdim_server** out) try {
   if (!c || !out || max_inflight < 1 || max_batch < 1) return fail(SDIM_EINVAL, "bad server arguments");
-  sdim_server* s = new sdim_server();
+  std::unique_ptr<sdim_server> s(new sdim_server());
   s->c = c;
   s->budget = hbm_budget_bytes;
   s->max_inflight = std::min<int>(max_inflight, (int)c->streams.size());
   s->max_batch = max_batch;
-  s->th = std::thread(serve_loop, s);
-  *out = s;
+  s->th = std::thread(serve_loop, s.get());
+  *out = s.release();
   return 0;
-}
+} SDIM_CATCH
 
-int sdim_server_submit(sdim_server* s, const sdim_job* j, uint64_t* request_id) {
+int sdim_server_submit(sdim_server* s, const sdim_job* j, uint64_t* request_id) try {
   if (!s || !j || !request_id) return fail(SDIM_EINVAL, "null argument");
   if (j->n < 1 || !supported_dim(j->d) || j->shots < 1 || j->n_ops < 0 || (j->n_ops && !j->ir))
     return fail(SDIM_EINVAL, "invalid request");
@@ -2101,7 +2152,7 @@ int sdim_server_submit(sdim_server* s, const sdim_job* j, uint64_t* request_id) 
   }
   s->cv.notify_all();
   return 0;
-}
+} SDIM_CATCH
 
 int sdim_server_poll(sdim_server* s, uint64_t id, int32_t* state) {
   if (!s || !state) return fail(SDIM_EINVAL, "null argument");

@@ -756,8 +756,12 @@ SDIM_DEV void sweep_groups(const Tab& V, const Scr& S, uint32_t nT, uint32_t nF,
   const uint32_t tid = threadIdx.x;
   const uint32_t rs = V.rs;
     if (nT && nF) {
+      // NRG threads per group.  With row blocks (il > 0) adjacent lanes take adjacent touched
+      // rows of one group, so touched rows of one block share a line within a load
+      // instruction; with contiguous rows (il = 0) adjacent lanes take adjacent groups of a row.
       uint32_t NRG, j0, g0, jstep;
       if (nF >= NT) { NRG = 1; j0 = tid; g0 = 0; jstep = NT; }
+      else if (V.il) { NRG = NT / nF; j0 = tid / NRG; g0 = tid % NRG; jstep = nF; }
       else { NRG = NT / nF; j0 = tid % nF; g0 = tid / nF; jstep = nF; }
       if (g0 < NRG) {
         for (uint32_t j = j0; j < nF; j += jstep) {
