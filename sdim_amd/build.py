@@ -23,7 +23,10 @@ GANG_DIMS = (2, 3, 5, 7, 11)
 
 def units():
     """(object name, source, extra defines) of every translation unit."""
-    out = [(f"batch_{nt}", "batch.hip", [f"-DSDIM_BATCH_NT={nt}"]) for nt in BATCH_SIZES]
+    # the one-wave kernel (launch class of small tableaux) gets 256 VGPRs, two waves per SIMD: its
+    # serial gate layers keep 2 x 16 rows of a column word in registers
+    out = [(f"batch_{nt}", "batch.hip", [f"-DSDIM_BATCH_NT={nt}"] + (["-DSDIM_BATCH_WAVES=2"] if nt == 64 else []))
+           for nt in BATCH_SIZES]
     out += [(f"gang_{d}", "gang.hip", [f"-DSDIM_GANG_D={d}"]) for d in GANG_DIMS]
     out += [("plain", "plain.hip", []), ("frame", "frame.hip", []), ("noalloc", "noalloc.hip", []),
             ("runtime", "runtime.cpp", [])]

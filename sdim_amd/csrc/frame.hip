@@ -95,26 +95,25 @@ SDIM_DEV void frame_walk(const FrameParams& P, const ItemDesc& it, uint8_t* F, u
         *fq = (uint8_t)((f & 15u) | (addm(f >> 4, negm(f & 15u, d), d) << 4));
         break;
       }
-      case FOP_CNOT: case FOP_CNOTI: {   // x_t += x_c, z_c -= z_t (inverse: signs flipped)
+      // two-qudit updates in the reference's statement order (sdim/program.py:105-112), each
+      // statement reading what the previous one wrote, so a gate whose two qudits coincide ends
+      // as the reference's in-place update does
+      case FOP_CNOT: case FOP_CNOTI: {   // x_t += x_c, then z_c -= z_t (inverse: signs flipped)
         uint8_t* ft = &F[(size_t)o.t * NT + tid];
-        const uint32_t fc = *fq, fT = *ft;
         const bool inv = kind == FOP_CNOTI;
-        const uint32_t xc = fc & 15u, zt = fT >> 4;
-        const uint32_t xt = addm(fT & 15u, inv ? negm(xc, d) : xc, d);
-        const uint32_t zc = addm(fc >> 4, inv ? zt : negm(zt, d), d);
-        *fq = (uint8_t)(xc | (zc << 4));
-        *ft = (uint8_t)(xt | (zt << 4));
+        uint32_t fc = *fq, fT = *ft;
+        *ft = (uint8_t)(addm(fT & 15u, inv ? negm(fc & 15u, d) : (fc & 15u), d) | (fT & 0xF0u));
+        fc = *fq; fT = *ft;
+        *fq = (uint8_t)((fc & 15u) | (addm(fc >> 4, inv ? (fT >> 4) : negm(fT >> 4, d), d) << 4));
         break;
       }
-      case FOP_CZ: case FOP_CZI: {       // z_t += x_c, z_c += x_t (inverse: -=)
+      case FOP_CZ: case FOP_CZI: {       // z_t += x_c, then z_c += x_t (inverse: -=)
         uint8_t* ft = &F[(size_t)o.t * NT + tid];
-        const uint32_t fc = *fq, fT = *ft;
         const bool inv = kind == FOP_CZI;
-        const uint32_t xc = fc & 15u, xt = fT & 15u;
-        const uint32_t zt = addm(fT >> 4, inv ? negm(xc, d) : xc, d);
-        const uint32_t zc = addm(fc >> 4, inv ? negm(xt, d) : xt, d);
-        *fq = (uint8_t)(xc | (zc << 4));
-        *ft = (uint8_t)(xt | (zt << 4));
+        uint32_t fc = *fq, fT = *ft;
+        *ft = (uint8_t)((fT & 15u) | (addm(fT >> 4, inv ? negm(fc & 15u, d) : (fc & 15u), d) << 4));
+        fc = *fq; fT = *ft;
+        *fq = (uint8_t)((fc & 15u) | (addm(fc >> 4, inv ? negm(fT & 15u, d) : (fT & 15u), d) << 4));
         break;
       }
       case FOP_SWAP: {

@@ -117,12 +117,17 @@ struct LaunchClass {
   int d = 0;          // gang classes: the dimension of their items
   int rb = 1;         // HBM-resident classes: sweep rows in flight per thread
   int opb = 2;        //                       gate batches per op load
+  uint32_t max_n = 0;  // > 0: only items of at most max_n qudits and max_s shots
+  uint32_t max_s = 0;
 };
 // default class table; the small HBM class is configurable (options "hbm_threads", "hbm_per_cu"),
 // and the HBM classes' sweep depth and op batching (options "sweep_rows", "seg_batches")
 std::vector<LaunchClass> make_classes(int hbm_nt, int hbm_per_cu, int rb, int opb) {
   auto budget = [](int per_cu) { return (uint32_t)((163840 / per_cu - 256) & ~15); };
   return {
+      // small tableaux: one-wave workgroups whose serial gate layers keep the rows in registers
+      // (the 64-thread kernel is built for two waves per SIMD, hence eight per CU)
+      {true, 64, budget(8), 8, 0, 0, 1, 2, SDIM_TINY_ROWS, SDIM_TINY_SHOTS},
       {true, 128, budget(8), 8},
       {true, 256, budget(4), 4},
       {true, 512, budget(2), 2},
@@ -279,6 +284,21 @@ int expand(const sdim_job& j, JobPlan& jp, std::vector<uint64_t>& keep, bool sna
     const bool two = (gid >= 9 && gid <= 13);
     if (two && (t < 0 || t >= n)) return fail(SDIM_EINVAL, "target index out of range at op " + std::to_string(k));
     const uint32_t Q = (uint32_t)q, Tq = (uint32_t)(two ? t : 0), src = (uint32_t)k;
+    // CZ, CZ^-1 and SWAP on one qudit: the closed forms assume two rows, so the reference's own
+    // words (sdim/tableau/tableau_gates.py:241-243, 258-260, 320-328) are lowered instead; SUM
+    // and SUM^-1 on one qudit already equal the reference's in-place update
+    if (two && Q == Tq && gid >= 11) {
+      if (gid == 13) {
+        push(OP_SUM, Q, Q, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0);
+        push(OP_SUM, Q, Q, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0);
+        push(OP_SUM, Q, Q, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0); push(OP_H, Q, 0, 0, 0, src, 0);
+      } else {
+        push(OP_HI, Q, 0, 0, 0, src, 0);
+        push(gid == 11 ? OP_SUM : OP_SUMI, Q, Q, 0, 0, src, 0);
+        push(OP_H, Q, 0, 0, 0, src, 0);
+      }
+      continue;
+    }
     switch (gid) {
       case 0: break;
       case 1: push(OP_PAULI, Q, 0, 1 % d, 0, src, 0); break;
@@ -1010,6 +1030,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       for (int k = 0; k < kNumClasses && cls < 0; k++) {
         const LaunchClass& L = kClasses[k];
         if (L.resident && !c->allow_lds) continue;
+        if (L.max_n && (n > L.max_n || S > L.max_s)) continue;
         if (need(k, S) <= std::min<uint32_t>(L.lds, cap)) cls = k;
       }
       if (cls < 0) {

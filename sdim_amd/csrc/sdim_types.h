@@ -26,6 +26,13 @@
 
 namespace sdim {
 
+// Tableaux of at most this many qudits run serial gate layers with their rows in registers
+// (one-wave workgroups of the launch class of small tableaux).
+constexpr uint32_t SDIM_TINY_ROWS = 16;
+// ... when an item holds at most this many shots (the class's one wave also runs the per-shot
+// phase passes)
+constexpr uint32_t SDIM_TINY_SHOTS = 64;
+
 enum OpKind : uint32_t {
   OP_H = 0, OP_HI = 1, OP_P = 2, OP_PI = 3, OP_SUM = 4, OP_SUMI = 5,
   OP_PAULI = 6,     // X^a Z^b conjugation, a = byte1, b = byte2 (shot-invariant)

@@ -341,6 +341,211 @@ SDIM_DEV bool two_rows(const DevOp& o) {
   return k == OP_SUM || k == OP_SUMI || k == OP_CZ || k == OP_CZI || k == OP_SWAP;
 }
 
+// ------------------------------------------------------------------ serial layers of small tableaux
+// A tableau of at most TINY_ROWS rows whose row has at most 64 column words: in a serial layer
+// the thread of word w keeps word w of every row in registers (a vector indexed by the op's
+// qudit, which is the same in every lane), so an op costs no LDS round trip.  The updates are
+// those of apply_odd / apply_bin on the register copies.  For d = 2 a row's G = 2n <= 32
+// generators sit in the low half of word 0; the high half holds padding columns, which are zero
+// from the start and stay zero (every update is column by column: gates and rowsums combine the
+// entries of one column, and a rowsum adds f_i times the pivot column with f_i = -x[q, i] = 0
+// there), so the register copy keeps 32 bits per row.
+constexpr uint32_t TINY_ROWS = SDIM_TINY_ROWS;
+typedef uint32_t RowsU32 __attribute__((ext_vector_type(TINY_ROWS)));
+
+// Each op reads rows q and t, computes their new words in the switch (an op leaves a row's word
+// alone unless it writes it) and writes row t, then row q, after it: the register vectors are
+// written in one place, so no branch of the switch holds a copy of them.  Row t before row q is
+// the order in which the LDS form stores them, so an op whose two qudits coincide ends the same.
+template <int D>
+SDIM_DEV uint32_t apply_odd_rows(const DevOp& o, RowsU32& X, RowsU32& Z, const Tab& T, uint32_t w, uint32_t& dl,
+                                 const Scr& S) {
+  const uint32_t q = o.q, t = o.t;
+  const uint32_t xq = X[q], zq = Z[q], xt = X[t], zt = Z[t];
+  uint32_t vxt = xt, vzt = zt, vxq = xq, vzq = zq, nb = 0;
+  bool wxq = false, wzq = false;
+  switch (op_kind(o)) {
+    case OP_H:                                     // (x, z) <- (-z, x); phase += po (-z) x
+      vxq = negm<D>(zq); vzq = xq; wxq = wzq = true;
+      dl = subm<D>(dl, mulv<D>(xq, zq));
+      nb = 16;
+      break;
+    case OP_HI:                                    // (x, z) <- (z, -x); phase += po z (-x)
+      vxq = zq; vzq = negm<D>(xq); wxq = wzq = true;
+      dl = subm<D>(dl, mulv<D>(xq, zq));
+      nb = 16;
+      break;
+    case OP_P: {                                   // z += x; phase += x (x - 1) / 2
+      const MulTab h = make_half_tab<D>();
+      vzq = addm<D>(zq, xq); wzq = true;
+      dl = addm<D>(dl, mult<D>(h, xq));
+      nb = 12;
+      break;
+    }
+    case OP_PI: {
+      const MulTab h = make_half_tab<D>();
+      vzq = subm<D>(zq, xq); wzq = true;
+      dl = subm<D>(dl, mult<D>(h, xq));
+      nb = 12;
+      break;
+    }
+    case OP_SUM:                                   // x[t] += x[c]; z[c] += (d - 1) z[t]
+      vxt = addm<D>(xt, xq);
+      vzq = subm<D>(zq, zt); wzq = true;
+      nb = 24;
+      break;
+    case OP_SUMI:
+      vxt = subm<D>(xt, xq);
+      vzq = addm<D>(zq, zt); wzq = true;
+      nb = 24;
+      break;
+    case OP_PAULI:                                 // X^a Z^b: phase += b x - a z
+      dl = addm<D>(dl, subm<D>(mult<D>(S.mtab[op_b(o)], xq), mult<D>(S.mtab[op_a(o)], zq)));
+      nb = 8;
+      break;
+    case OP_CZ:                                    // z[t] += x[c], z[c] += x[t]; phase += x[c] x[t]
+      vzt = addm<D>(zt, xq);
+      vzq = addm<D>(zq, xt); wzq = true;
+      dl = addm<D>(dl, mulv<D>(xq, xt));
+      nb = 24;
+      break;
+    case OP_CZI:
+      vzt = subm<D>(zt, xq);
+      vzq = subm<D>(zq, xt); wzq = true;
+      dl = subm<D>(dl, mulv<D>(xq, xt));
+      nb = 24;
+      break;
+    case OP_SWAP:
+      vxq = xt; vzq = zt; vxt = xq; vzt = zq; wxq = wzq = true;
+      nb = 32;
+      break;
+    case OP_NOISE: {                               // snapshot of row q for the end-of-segment pass
+      uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
+      ((uint32_t*)snap)[T.wofs + w] = xq;
+      ((uint32_t*)(snap + T.rs_full))[T.wofs + w] = zq;
+      nb = 8;
+      break;
+    }
+  }
+  if (q == t) {                                    // row q holds what was just written to row t
+    if (!wxq) vxq = vxt;
+    if (!wzq) vzq = vzt;
+  }
+  X[t] = vxt; Z[t] = vzt;
+  X[q] = vxq; Z[q] = vzq;
+  return nb;
+}
+
+SDIM_DEV uint32_t apply_bin_rows(const DevOp& o, RowsU32& X, RowsU32& Z, const Tab& T, uint32_t w, uint64_t& d0,
+                                 uint64_t& d1, const Scr& S) {
+  const uint32_t q = o.q, t = o.t;
+  const uint32_t xq = X[q], zq = Z[q], xt = X[t], zt = Z[t];
+  uint32_t vxt = xt, vzt = zt, vxq = xq, vzq = zq, nb = 0;
+  bool wxq = false, wzq = false;
+  switch (op_kind(o)) {
+    case OP_H:
+    case OP_HI:                                    // swap; phase += 2 x z
+      vxq = zq; vzq = xq; wxq = wzq = true;
+      d1 ^= (uint64_t)(xq & zq);
+      nb = 32;
+      break;
+    case OP_P:                                     // phase += x; z ^= x
+      vzq = zq ^ xq; wzq = true;
+      d1 ^= d0 & xq; d0 ^= xq;
+      nb = 24;
+      break;
+    case OP_PI:                                    // phase += 3 x; z ^= x
+      vzq = zq ^ xq; wzq = true;
+      d1 ^= (uint64_t)xq & ~d0; d0 ^= xq;
+      nb = 24;
+      break;
+    case OP_SUM:
+    case OP_SUMI:
+      vxt = xt ^ xq;
+      vzq = zq ^ zt; wzq = true;
+      nb = 48;
+      break;
+    case OP_PAULI:                                 // phase += 2 (b x - a z)
+      d1 ^= (uint64_t)(((op_b(o) & 1) ? xq : 0u) ^ ((op_a(o) & 1) ? zq : 0u));
+      nb = 16;
+      break;
+    case OP_CZ:
+    case OP_CZI:                                   // z[t] ^= x[c], z[c] ^= x[t]; phase += 2 x[c] x[t]
+      vzt = zt ^ xq;
+      vzq = zq ^ xt; wzq = true;
+      d1 ^= (uint64_t)(xq & xt);
+      nb = 48;
+      break;
+    case OP_SWAP:
+      vxq = xt; vzq = zt; vxt = xq; vzt = zq; wxq = wzq = true;
+      nb = 64;
+      break;
+    case OP_NOISE: {
+      uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
+      ((uint64_t*)snap)[T.wofs + w] = xq;
+      ((uint64_t*)(snap + T.rs_full))[T.wofs + w] = zq;
+      nb = 16;
+      break;
+    }
+  }
+  if (q == t) {
+    if (!wxq) vxq = vxt;
+    if (!wzq) vzq = vzt;
+  }
+  X[t] = vxt; Z[t] = vzt;
+  X[q] = vxq; Z[q] = vzq;
+  return nb;
+}
+
+// One serial layer (L ops in program order) of a view with V.n <= TINY_ROWS rows and NWd <= 64
+// words, run by the first wave: lane w loads word w of every row, applies the ops, stores the
+// rows back.  Ops are fetched 64 per global round trip (one per lane) and broadcast by readlane.
+// Phase increments accumulate in dl (odd d) or d0 / d1 (d = 2) as in the LDS form.
+template <int D>
+SDIM_DEV void ser_layer_rows(const DevOp* lops, uint32_t L, const Tab& V, uint32_t NWd, const Scr& S,
+                             uint32_t& dl, uint64_t& d0, uint64_t& d1, unsigned long long& gb) {
+  constexpr bool BIN = (D == 2);
+  if (threadIdx.x >= 64) return;
+  const uint32_t lane = threadIdx.x, w = lane, n = V.n;
+  const bool mine = w < NWd;
+  RowsU32 X = (RowsU32)0, Z = (RowsU32)0;
+  if (mine) {
+    #pragma unroll
+    for (uint32_t r = 0; r < TINY_ROWS; r++) {
+      if (r < n) {
+        if constexpr (BIN) { X[r] = (uint32_t)*w64(V.X, r, w, V.rs, V.il); Z[r] = (uint32_t)*w64(V.Z, r, w, V.rs, V.il); }
+        else { X[r] = *w32(V.X, r, w, V.rs, V.il); Z[r] = *w32(V.Z, r, w, V.rs, V.il); }
+      }
+    }
+  }
+  uint32_t nw0 = 0, nq = 0, ntg = 0;
+  if (lane < L) { const DevOp& src = lops[lane]; nw0 = src.w0; nq = src.q; ntg = src.t; }
+  for (uint32_t c0 = 0; c0 < L; c0 += 64) {
+    const uint32_t cw0 = nw0, cq = nq, ctg = ntg;
+    if (c0 + 64 + lane < L) { const DevOp& src = lops[c0 + 64 + lane]; nw0 = src.w0; nq = src.q; ntg = src.t; }
+    const uint32_t cnt = (L - c0 < 64) ? L - c0 : 64;
+    for (uint32_t b = 0; b < cnt; b++) {
+      DevOp o;
+      o.w0 = (uint32_t)__builtin_amdgcn_readlane((int)cw0, (int)b);
+      o.q = (uint32_t)__builtin_amdgcn_readlane((int)cq, (int)b);
+      o.t = (uint32_t)__builtin_amdgcn_readlane((int)ctg, (int)b);
+      o.src = 0;
+      if (!mine) continue;
+      if constexpr (BIN) gb += apply_bin_rows(o, X, Z, V, w, d0, d1, S);
+      else gb += apply_odd_rows<D>(o, X, Z, V, w, dl, S);
+    }
+  }
+  if (mine) {
+    #pragma unroll
+    for (uint32_t r = 0; r < TINY_ROWS; r++) {
+      if (r < n) {
+        if constexpr (BIN) { *w64(V.X, r, w, V.rs, V.il) = (uint64_t)X[r]; *w64(V.Z, r, w, V.rs, V.il) = (uint64_t)Z[r]; }
+        else { *w32(V.X, r, w, V.rs, V.il) = X[r]; *w32(V.Z, r, w, V.rs, V.il) = Z[r]; }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ gate segment
 // The layers of one gate segment over the column words [0, NWd) of view V (the whole
 // tableau, or one column tile staged in LDS); ops[kl] is the first layer header.  Returns
@@ -381,7 +586,11 @@ SDIM_DEV uint32_t seg_layers(const DevOp* ops, uint32_t kl, uint32_t nl, const T
     // fetches 64 ops per global round trip, one per lane, the next 64 in flight, and
     // broadcasts them with readlane.
     const bool ser = lh.t != 0;
-    if (ser) {
+    // one-wave workgroups (the launch class of small tableaux) keep a small tableau's rows in
+    // registers through a serial layer
+    if (NT == 64 && ser && V.n <= TINY_ROWS && NWd <= 64) {
+      ser_layer_rows<D>(lops, L, V, NWd, S, dl, b0, b1, gb);
+    } else if (ser) {
       const uint32_t nwv = ((NWd < NT ? NWd : NT) + 63) / 64;
       if (tid / 64 < nwv) {
         const uint32_t lane = tid & 63;

@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value")
     ap.add_argument("--which", type=int, default=0, help="selftest case")
     ap.add_argument("--expect-error", default="", help="random case: every submission must fail with this message")
+    ap.add_argument("--same", type=float, default=0.0,
+                    help="random case: share of two-qudit gates whose two qudits coincide")
     a = ap.parse_args()
     os.environ.setdefault("SDIM_SIM_CUS", "256" if a.case == "uneven" else "4")
     from tests.cpu_sim.sim_engine import SimEngine
@@ -88,7 +90,7 @@ def main():
     if a.case == "random":
         rng = np.random.default_rng(a.seed)
         for d in (2, 3, 5, 7, 11):
-            jobs = [rand_job(rng, int(rng.integers(1, 14)), d, int(rng.integers(1, 70)), stream=i)
+            jobs = [rand_job(rng, int(rng.integers(1, 14)), d, int(rng.integers(1, 70)), stream=i, same=a.same)
                     for i in range(a.count)]
             variants = [("batched", dict(opts)), ("plain", dict(opts))]
             if a.hbm:

@@ -7,7 +7,9 @@ import numpy as np
 from oracle import c_oracle, philox
 
 
-def rand_job(rng, n, d, depth, meas=0.15, noise=0.1, shots=3, stream=0):
+def rand_job(rng, n, d, depth, meas=0.15, noise=0.1, shots=3, stream=0, same=0.0):
+    """Random circuit over every gate id; `same` is the share of two-qudit gates whose two qudits
+    coincide (the reference accepts them)."""
     from sdim_amd.engine import Job
     ops, ch, pr = [], [], []
     for _ in range(depth):
@@ -16,6 +18,9 @@ def rand_job(rng, n, d, depth, meas=0.15, noise=0.1, shots=3, stream=0):
             ops.append((int(rng.choice([14, 15, 16])), int(rng.integers(n)), -1)); ch.append(0); pr.append(0.0)
         elif r < meas + noise:
             ops.append((17, int(rng.integers(n)), -1)); ch.append(int(rng.integers(3))); pr.append(float(rng.random()))
+        elif same > 0 and r < meas + noise + same * (1 - meas - noise):
+            a = int(rng.integers(n))
+            ops.append((int(rng.integers(9, 14)), a, a)); ch.append(0); pr.append(0.0)
         elif r < 0.65 or n < 2:
             ops.append((int(rng.integers(0, 9)), int(rng.integers(n)), -1)); ch.append(0); pr.append(0.0)
         else:

@@ -66,6 +66,18 @@ def test_serial_layers_match_oracle():
     assert r.returncode == 0 and ASAN_REPORT not in out, out[-6000:]
 
 
+def test_two_qudit_gates_on_one_qudit_match_oracle():
+    """SUM, SUM^-1, CZ, CZ^-1 and SWAP whose two qudits coincide (the reference accepts them; CZ
+    and SWAP then follow the reference's literal words), with serial layers (rows in registers
+    for tableaux of at most 16 qudits) and without, in LDS and HBM, batched and plain."""
+    r, out = _run("plain", "--case", "random", "--seed", "31", "--count", "6", "--hbm", "--same", "0.4",
+                  "--opt", "serial_max=8")
+    assert r.returncode == 0, out[-4000:]
+    r, out = _run("plain", "--case", "random", "--seed", "32", "--count", "6", "--same", "0.4",
+                  "--opt", "serial_max=0")
+    assert r.returncode == 0, out[-4000:]
+
+
 def test_address_sanitizer_random_circuits():
     r, out = _run("asan", "--case", "random", "--seed", "12", "--count", "4", "--hbm")
     assert r.returncode == 0 and ASAN_REPORT not in out, out[-6000:]

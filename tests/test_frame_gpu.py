@@ -67,6 +67,20 @@ def test_frame_random_circuits(eng, d):
     tk.release()
 
 
+@pytest.mark.parametrize("d", [2, 3, 11])
+def test_frame_two_qudit_gates_on_one_qudit(eng, d):
+    """CNOT, CZ (and inverses) and SWAP whose two qudits coincide update the frame as the
+    reference's in-place statements do."""
+    rng = np.random.default_rng(2200 + d)
+    jobs = [rand_job(rng, int(n), d, 8 * int(n), shots=int(s), stream=i, same=0.4)
+            for i, (n, s) in enumerate([(1, 9), (6, 200), (30, 65)])]
+    refs = _reference(eng, jobs)
+    print(f"[frame] same-qudit d={d}", flush=True)
+    tk = eng.frame_run(jobs, refs, shot0=1)
+    _check(tk, jobs, refs, 1)
+    tk.release()
+
+
 @pytest.mark.parametrize("d", [2, 5])
 def test_frame_device_memory_frames(d):
     """Frames in device memory (no LDS budget, and n above the default budget) equal LDS frames."""

@@ -30,6 +30,24 @@ def test_random_circuits_small(engine, mode, d):
     tk.release()
 
 
+@pytest.mark.parametrize("mode,opts", [("batched", dict()), ("batched", dict(serial_max=1000)),
+                                       ("batched", dict(serial_max=0, allow_lds=0)), ("plain", dict())])
+def test_two_qudit_gates_on_one_qudit(mode, opts):
+    """SUM, SUM^-1, CZ, CZ^-1 and SWAP whose two qudits coincide, among random circuits of every
+    dimension: serial layers of small tableaux (rows in registers), parallel layers, HBM items,
+    the plain path."""
+    from sdim_amd.engine import Engine
+    eng = Engine(**opts)
+    rng = np.random.default_rng(700 + len(opts) + (mode == "plain"))
+    jobs = [rand_job(rng, int(rng.integers(1, 20)), d, int(rng.integers(10, 80)), stream=i, same=0.4)
+            for i, d in enumerate([2, 3, 5, 7, 11] * 3)]
+    print(f"[parity-same-qudit] mode={mode} opts={opts}", flush=True)
+    tk = eng.run(jobs, mode=mode)
+    check_ticket(tk, jobs)
+    tk.release()
+    eng.close()
+
+
 @pytest.mark.parametrize("opts", [dict(), dict(allow_lds=0), dict(shots_per_item=1), dict(threads=256),
                                   dict(threads=512, allow_lds=0), dict(serial_max=0),
                                   dict(serial_max=1000, allow_lds=0), dict(interleave=0, allow_lds=0),
