@@ -428,6 +428,7 @@ def main():
         opts["threads"] = args.threads
     tuned = None
     options_source = "defaults"
+    base_opts = dict(opts)       # the other legs are other workloads: no options tuned for this one
     from sdim_amd import autotune
     if args.tune:
         tuned = autotune.tune(jobs, reps=max(1, min(args.steps, 3)), max_seconds=150.0)
@@ -495,7 +496,8 @@ def main():
                    "threads": eng.get_option("threads"), "shots_per_item": eng.get_option("shots_per_item"),
                    "options": ({k: eng.get_option(k) for k in tuned["options"]} if tuned else "defaults"),
                    "options_source": options_source,
-                   "tuned_from_ms": tuned["default_ms"] if tuned else None},
+                   "tuned_from_ms": tuned["default_ms"] if tuned else None,
+                   "other_legs_options": base_opts or "defaults"},
         "device_ms_per_step": dev_ms / args.steps,
         "first_submit_s": t_submit,
         "roofline": roof("hbm", HBM_PEAK_GBS) or roof("lds", LDS_PEAK_GBS),
@@ -541,18 +543,19 @@ def main():
                                     "items": st1["items"], "xz_shared_by_shots": 1,
                                     "config": "shots_per_item=1: one tableau (X, Z and phases) per shot"}
 
+    leg_eng = Engine(**base_opts) if base_opts != opts else eng
     if not args.no_large and args.workload == "uneven":
-        out["large_tableau"] = large_tableau(eng, max(1, min(args.steps, 5)), 1)
+        out["large_tableau"] = large_tableau(leg_eng, max(1, min(args.steps, 5)), 1)
 
     if not args.no_frame and args.workload == "uneven":
-        out["frame"] = frame_sampler(eng, max(1, min(args.steps, 10)), 1, args.cpu_seconds)
+        out["frame"] = frame_sampler(leg_eng, max(1, min(args.steps, 10)), 1, args.cpu_seconds)
 
     if not args.no_published and args.workload == "uneven":
-        out["published_workload"] = published_workload(eng, max(1, min(args.steps, 5)),
+        out["published_workload"] = published_workload(leg_eng, max(1, min(args.steps, 5)),
                                                         0 if args.no_cpu else args.cpu_seconds)
 
     if not args.no_serve:
-        out["serving"] = serving(eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)
+        out["serving"] = serving(leg_eng, args.serve_requests, args.serve_rate, args.serve_budget_gb)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_seconds)

@@ -78,7 +78,9 @@ const char* sdim_last_error(void);
 /*
  * Tuning / engine options (no reference counterpart).  Keys:
  *   "threads"         workgroup size of the batch kernel: 0 (per launch class), 256, 512 or 1024
- *   "lds_bytes"       upper bound of the LDS budget per workgroup (<= 163584)
+ *   "lds_bytes"       upper bound of the LDS budget per workgroup of the one-workgroup launch
+ *                     classes (16384..163584): the LDS/HBM cut-over; gang members always take a
+ *                     CU's whole LDS
  *   "shots_per_item"  shots that share one X/Z block per work item (0 = automatic: enough
  *                     items to fill the device twice, but at least min(64, n/16) shots per
  *                     item; 1 = a separate tableau per shot)

@@ -36,7 +36,7 @@ SPACE = [
     ("seg_batches", (2, 1)),
     ("serial_max", (4, 0, 2, 8, 16)),
     ("shots_per_item", (0, 1, 2, 4, 8, 16, 32)),
-    ("lds_bytes", (163584, 131072, 98304, 65536)),
+    ("lds_bytes", (163584, 131072, 98304, 65536, 49152, 40960, 32768, 28672, 24576, 20480)),
     ("gang_min_n", (1536, 1024, 2048, 4096, 0)),
     ("gang_chunk", (128, 256, 64)),
     ("gang_threads", (512, 256, 1024)),
@@ -125,7 +125,7 @@ def tune(jobs, reps=3, space=SPACE, cache=None, verbose=False, max_seconds=120.0
             trials.append({"options": cand, "ms": ms})
             if verbose:
                 print(f"[tune] {key}={v}: {ms:.3f} ms (best {best_ms:.3f})", flush=True)
-            if ms < best_ms * 0.98:
+            if ms < best_ms * 0.99:          # graph replays repeat within about 0.5%
                 best, best_ms = cand, ms
     out = {"options": best, "ms": best_ms, "default_ms": default_ms, "trials": trials,
            "signature": signature(jobs)}

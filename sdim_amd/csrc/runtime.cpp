@@ -686,7 +686,8 @@ int enqueue(Ticket* T, int which, hipEvent_t gang_gate = nullptr) {
       Q.order = P.order + g.first;
       Q.n_items = g.count;
       Q.queue = T->d_queue + (&g - T->groups.data());
-      Q.lds_bytes = std::min<uint32_t>(L.lds, P.lds_bytes);
+      // lds_bytes caps the one-workgroup classes; a gang member takes a CU's whole LDS
+      Q.lds_bytes = L.gang ? L.lds : std::min<uint32_t>(L.lds, P.lds_bytes);
       if (L.gang) {
         if (gang_gate && hipStreamWaitEvent(T->stream, gang_gate, 0) != hipSuccess)
           return fail(SDIM_EHIP, "hipStreamWaitEvent failed");
@@ -1008,8 +1009,8 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
       if (K >= 2) {
         const uint32_t bpw = d == 2 ? 8 : 4;
         const uint32_t cb = gang_words(rs / bpw, K, bpw) * bpw;
-        const uint32_t lds = std::min<uint32_t>(163584, cap);
-        uint32_t Sg = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
+        const uint32_t lds = 163584;      // not capped by lds_bytes (the one-workgroup classes' cut-over)
+        uint32_t Sg =(uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
         auto gneed = [&](uint32_t s) -> uint64_t {
           return (uint64_t)scratch_bytes(n, cb, s, (uint32_t)gnt) + (has_noise ? noise_bytes(1, cb, s) : 0);
         };

@@ -65,7 +65,9 @@ def test_gang_sizes_agree_with_one_workgroup(d):
     for opts in (dict(gang_min_n=0), dict(gang_min_n=256, gang_chunk=16, gang_max=2),
                  dict(gang_min_n=256, gang_chunk=16, gang_max=8), dict(gang_min_n=256, gang_chunk=16, gang_max=64),
                  # serial layers; for d = 5 a member owns more column words than it has threads
-                 dict(gang_min_n=256, gang_chunk=16, gang_max=2, gang_threads=256, serial_max=1000)):
+                 dict(gang_min_n=256, gang_chunk=16, gang_max=2, gang_threads=256, serial_max=1000),
+                 # the LDS cap of the one-workgroup classes leaves gang members their whole LDS
+                 dict(gang_min_n=256, gang_chunk=16, gang_max=8, lds_bytes=32768)):
         print(f"[gang] sizes d={d} opts={opts}", flush=True)
         eng = Engine(**opts)
         tk = eng.run([job])

@@ -30,6 +30,32 @@ def test_random_circuits_small(engine, mode, d):
     tk.release()
 
 
+@pytest.mark.parametrize("mode,opts", [("batched", dict()), ("batched", dict(serial_max=8)),
+                                       ("batched", dict(allow_lds=0)), ("plain", dict())])
+def test_small_class_boundary_and_trivial_circuits(mode, opts):
+    """n = 15, 16, 17 around the limit of the small-tableau launch class (with up to 65 shots,
+    around its shot limit), an empty circuit, a measurement-only and a noise-only circuit (noise
+    probabilities 1, 0 and 1/2), every dimension."""
+    from sdim_amd.engine import Engine, Job
+    rng = np.random.default_rng(9)
+    jobs = []
+    for d in (2, 3, 5, 7, 11):
+        for n in (15, 16, 17):
+            jobs.append(rand_job(rng, n, d, 120, shots=int(rng.integers(1, 66)), stream=len(jobs), same=0.1))
+        jobs.append(Job(3, d, np.zeros((0, 3), np.int64), shots=2, seed=1, stream=len(jobs)))
+        jobs.append(Job(4, d, np.asarray([(14, q, -1) for q in range(4)] * 2, np.int64), shots=3, seed=2,
+                        stream=len(jobs)))
+        jobs.append(Job(3, d, np.asarray([(17, q % 3, -1) for q in range(9)], np.int64), shots=5, seed=3,
+                        stream=len(jobs), noise_channel=np.asarray([q % 3 for q in range(9)], np.int32),
+                        noise_prob=np.asarray([1.0, 0.0, 0.5] * 3)))
+    eng = Engine(**opts)
+    print(f"[parity-boundary] mode={mode} opts={opts}", flush=True)
+    tk = eng.run(jobs, mode=mode)
+    check_ticket(tk, jobs)
+    tk.release()
+    eng.close()
+
+
 @pytest.mark.parametrize("mode,opts", [("batched", dict()), ("batched", dict(serial_max=1000)),
                                        ("batched", dict(serial_max=0, allow_lds=0)), ("plain", dict())])
 def test_two_qudit_gates_on_one_qudit(mode, opts):
