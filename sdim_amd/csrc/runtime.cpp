@@ -1913,6 +1913,7 @@ struct Request {
   std::vector<uint8_t> records;  // MEASUREMENT_DTYPE bytes
   int64_t n_records = 0;
   uint64_t gate_shots = 0;
+  uint64_t cost = 0;             // work estimate: ops x n x (64-shot blocks), see serve_cost
   std::string error;
 };
 
@@ -1921,6 +1922,17 @@ struct Batch {
   std::vector<uint64_t> reqs;
   uint64_t bytes = 0;
 };
+
+// Serving work estimate of a request: its items repeat the circuit's X/Z evolution (ops x row
+// bytes ~ ops x n) once per block of at most 64 shots.
+uint64_t serve_cost(uint64_t ops, int64_t n, int64_t shots) {
+  return std::max<uint64_t>(1, ops) * (uint64_t)std::max<int64_t>(1, n) * (uint64_t)((shots + 63) / 64);
+}
+// requests of at most this estimate (about n = 300 with depth 8n and at most 64 shots) are small:
+// the last free in-flight slot of the server only takes a batch of them
+constexpr uint64_t kServeSmallCost = 1ull << 20;
+// requests batched together differ in estimate by at most this factor
+constexpr uint64_t kServeCostWindow = 16;
 
 uint64_t estimate_footprint(const sdim_job& j) {
   const uint32_t n = (uint32_t)j.n, d = (uint32_t)j.d;
@@ -2057,28 +2069,50 @@ void serve_loop(sdim_server* s) {
     s->inflight.swap(still);
     s->n_inflight = (int64_t)s->inflight.size();
 
-    // ---- admission: FIFO while the footprint fits the budget
+    // ---- admission: arrival order while the footprint fits the budget.  A batch's records come
+    // back when its last item finishes, so a batch holds requests of similar cost (within
+    // kServeCostWindow of its first request; the others stay pending, in order, for a later
+    // batch), and the last free in-flight slot only starts a batch of small requests, so that a
+    // small request does not wait until large work releases a slot.
     std::vector<uint64_t> take;
     uint64_t take_bytes = 0;
     bool quit = false, rejected = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       if (s->stop && s->pending.empty() && s->inflight.empty()) quit = true;
-      if (!quit && (int)s->inflight.size() < s->max_inflight) {
-        while (!s->pending.empty() && (int)take.size() < s->max_batch) {
-          Request& r = s->reqs[s->pending.front()];
+      const int free_slots = s->max_inflight - (int)s->inflight.size();
+      if (!quit && free_slots > 0) {
+        const bool small_only = free_slots == 1 && s->max_inflight > 1;
+        std::deque<uint64_t> rest;
+        uint64_t c0 = 0;
+        bool blocked = false;          // a request did not fit the budget: later ones wait behind it
+        while (!s->pending.empty()) {
+          const uint64_t id = s->pending.front();
+          s->pending.pop_front();
+          Request& r = s->reqs[id];
           if (r.footprint > s->budget) {       // can never fit: fail it
             r.state = 3;
             r.error = "request larger than the HBM budget";
-            s->pending.pop_front();
             rejected = true;
             continue;
           }
-          if (s->used + take_bytes + r.footprint > s->budget) break;
+          const bool same_class = take.empty()
+                                      ? (!small_only || r.cost <= kServeSmallCost)
+                                      : (r.cost <= c0 * kServeCostWindow && c0 <= r.cost * kServeCostWindow);
+          if (blocked || (int)take.size() >= s->max_batch || !same_class) {
+            rest.push_back(id);
+            continue;
+          }
+          if (s->used + take_bytes + r.footprint > s->budget) {
+            blocked = true;
+            rest.push_back(id);
+            continue;
+          }
+          if (take.empty()) c0 = r.cost;
           take_bytes += r.footprint;
-          take.push_back(r.id);
-          s->pending.pop_front();
+          take.push_back(id);
         }
+        s->pending.swap(rest);
         s->used += take_bytes;
       }
       if (rejected) s->cv.notify_all();
@@ -2157,6 +2191,7 @@ int sdim_server_submit(sdim_server* s, const sdim_job* j, uint64_t* request_id) 
   uint64_t ops = 0;
   for (int64_t k = 0; k < j->n_ops; k++) ops += (j->ir[3 * k] != 0);
   r.gate_shots = ops * (uint64_t)j->shots;
+  r.cost = serve_cost(ops, j->n, j->shots);
   r.t_arrive = now_s();
   {
     std::lock_guard<std::mutex> lk(s->mu);
