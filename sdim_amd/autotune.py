@@ -5,7 +5,8 @@ per-item work: the workgroup size and workgroups per CU of the HBM-resident
 launch class (`hbm_threads`, `hbm_per_cu`), the rows each thread keeps in flight
 in the rowsum sweep and the gate batches per op load of that class
 (`sweep_rows`, `seg_batches`: compile-time variants of the kernel picked per
-launch), shots per work item (`shots_per_item`), the LDS/HBM cut-over
+launch), serial gate layers (`serial_max`), the row interleave of the X/Z blocks
+(`interleave`), shots per work item (`shots_per_item`), the LDS/HBM cut-over
 (`lds_bytes`) and, for large circuits, the gang shape (`gang_min_n`,
 `gang_chunk`, `gang_threads`).
 `tune(jobs)` times device replays of the same submission under candidate
@@ -35,13 +36,14 @@ SPACE = [
     ("sweep_rows", (1, 2, 4)),
     ("seg_batches", (2, 1)),
     ("serial_max", (4, 0, 2, 8, 16)),
+    ("interleave", (3, 2)),
     ("shots_per_item", (0, 1, 2, 4, 8, 16, 32)),
     ("lds_bytes", (163584, 131072, 98304, 65536, 49152, 40960, 32768, 28672, 24576, 20480)),
     ("gang_min_n", (1536, 1024, 2048, 4096, 0)),
     ("gang_chunk", (128, 256, 64)),
     ("gang_threads", (512, 256, 1024)),
 ]
-DEFAULTS = {"hbm_threads": 256, "hbm_per_cu": 4, "sweep_rows": 1, "seg_batches": 2, "serial_max": 4,
+DEFAULTS = {"hbm_threads": 256, "hbm_per_cu": 4, "sweep_rows": 1, "seg_batches": 2, "serial_max": 4, "interleave": 3,
             "shots_per_item": 0,
             "lds_bytes": 163584, "gang_min_n": 1536, "gang_chunk": 128, "gang_threads": 512}
 # options that only matter for some workloads
