@@ -480,7 +480,7 @@ def main():
                 "unit": "GB/s", "frac": round(ach / peak, 4), "traffic": traffic,
                 "traffic_source": "profiles/batch_kernel_uneven_batch.md" if traffic else None,
                 "kernel": ("sdim::batch_kernel<%d>" % (eng.get_option("threads") or eng.get_option("hbm_threads"))
-                           if cls == "hbm" else "sdim::batch_kernel<128..1024> (LDS-resident classes)"),
+                           if cls == "hbm" else "sdim::batch_kernel<NT, true> (LDS-resident classes)"),
                 "items": c["items"], "algorithmic_bytes": c["bytes"], "kernel_ms": round(c["ms"], 3)}
 
     out = {
