@@ -405,7 +405,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus above 1 is not supported "
-                                   "(sharding is described in DESIGN.md, not built)"}))
+                                   "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
     cpu_par = None
     if not args.no_cpu and args.workload != "frame":

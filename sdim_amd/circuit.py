@@ -1,7 +1,7 @@
 """Circuit construction API, matching sdim's ``Circuit`` / ``CircuitInstruction``
 (sdim/circuit.py:5-242).
 
-Kept reference behaviours (each documented in DESIGN.md):
+Kept reference behaviours:
   * ``add_gate`` broadcasts one control over many targets, many controls over
     one target, or pairs equal-length lists (sdim/circuit.py:95-125);
   * two-qudit ``add_gate`` calls drop keyword parameters (circuit.py:125);
