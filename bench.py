@@ -23,7 +23,12 @@ Besides the headline (batched persistent path) the line carries:
                  configuration: a noiseless tableau reference shot + 4095 Pauli-frame
                  shots on the device, alone and as 64 circuits per launch, with the
                  frame oracle (C, 1 core) beside it
+  published_workload  the reference's published 100,000-gate circuits (d in {2,3,5,7},
+                 n in 2..14, one shot): all 52 batched, and n = 2 / 14 alone beside the
+                 published seconds
   serving        Poisson arrivals with HBM-budget admission: p50/p99 latency, aggregate rate
+                 (options tuned for the headline workload apply to it only; the other legs
+                 run with the engine's defaults)
   cpu_baseline   the CPU oracle (literal restatement of the reference's
                  Program.simulate tableau path, same interpreted cost structure),
                  timed on a bounded sample on this host, 1 core
