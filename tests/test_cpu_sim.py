@@ -76,6 +76,10 @@ def test_two_qudit_gates_on_one_qudit_match_oracle():
     r, out = _run("plain", "--case", "random", "--seed", "32", "--count", "6", "--same", "0.4",
                   "--opt", "serial_max=0")
     assert r.returncode == 0, out[-4000:]
+    # one-wave HBM-resident workgroups: the register form of serial layers on LDS column tiles
+    r, out = _run("plain", "--case", "random", "--seed", "41", "--count", "6", "--same", "0.2",
+                  "--opt", "hbm_threads=64", "--opt", "allow_lds=0", "--opt", "serial_max=8")
+    assert r.returncode == 0, out[-4000:]
 
 
 def test_address_sanitizer_random_circuits():
