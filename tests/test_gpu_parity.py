@@ -57,7 +57,9 @@ def test_small_class_boundary_and_trivial_circuits(mode, opts):
 
 
 @pytest.mark.parametrize("mode,opts", [("batched", dict()), ("batched", dict(serial_max=1000)),
-                                       ("batched", dict(serial_max=0, allow_lds=0)), ("plain", dict())])
+                                       ("batched", dict(serial_max=0, allow_lds=0)),
+                                       ("batched", dict(hbm_threads=64, allow_lds=0, serial_max=8)),
+                                       ("plain", dict())])
 def test_two_qudit_gates_on_one_qudit(mode, opts):
     """SUM, SUM^-1, CZ, CZ^-1 and SWAP whose two qudits coincide, among random circuits of every
     dimension: serial layers of small tableaux (rows in registers), parallel layers, HBM items,
