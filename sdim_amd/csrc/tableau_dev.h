@@ -438,54 +438,47 @@ SDIM_DEV uint32_t apply_odd_rows(const DevOp& o, RowsU32& X, RowsU32& Z, const T
 
 SDIM_DEV uint32_t apply_bin_rows(const DevOp& o, RowsU32& X, RowsU32& Z, const Tab& T, uint32_t w, uint64_t& d0,
                                  uint64_t& d1, const Scr& S) {
-  const uint32_t q = o.q, t = o.t;
+  const uint32_t q = o.q, t = o.t, k = op_kind(o);
   const uint32_t xq = X[q], zq = Z[q], xt = X[t], zt = Z[t];
   uint32_t vxt = xt, vzt = zt, vxq = xq, vzq = zq, nb = 0;
   bool wxq = false, wzq = false;
-  switch (op_kind(o)) {
-    case OP_H:
-    case OP_HI:                                    // swap; phase += 2 x z
-      vxq = zq; vzq = xq; wxq = wzq = true;
-      d1 ^= (uint64_t)(xq & zq);
-      nb = 32;
-      break;
-    case OP_P:                                     // phase += x; z ^= x
-      vzq = zq ^ xq; wzq = true;
-      d1 ^= d0 & xq; d0 ^= xq;
-      nb = 24;
-      break;
-    case OP_PI:                                    // phase += 3 x; z ^= x
-      vzq = zq ^ xq; wzq = true;
-      d1 ^= (uint64_t)xq & ~d0; d0 ^= xq;
-      nb = 24;
-      break;
-    case OP_SUM:
-    case OP_SUMI:
-      vxt = xt ^ xq;
-      vzq = zq ^ zt; wzq = true;
-      nb = 48;
-      break;
-    case OP_PAULI:                                 // phase += 2 (b x - a z)
-      d1 ^= (uint64_t)(((op_b(o) & 1) ? xq : 0u) ^ ((op_a(o) & 1) ? zq : 0u));
-      nb = 16;
-      break;
-    case OP_CZ:
-    case OP_CZI:                                   // z[t] ^= x[c], z[c] ^= x[t]; phase += 2 x[c] x[t]
-      vzt = zt ^ xq;
-      vzq = zq ^ xt; wzq = true;
-      d1 ^= (uint64_t)(xq & xt);
-      nb = 48;
-      break;
-    case OP_SWAP:
-      vxq = xt; vzq = zt; vxt = xq; vzt = zq; wxq = wzq = true;
-      nb = 64;
-      break;
-    case OP_NOISE: {
-      uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
-      ((uint64_t*)snap)[T.wofs + w] = xq;
-      ((uint64_t*)(snap + T.rs_full))[T.wofs + w] = zq;
-      nb = 16;
-      break;
+  if (k <= OP_SUMI) {
+    // the common gates without a branch per kind: H, H^-1 swap x and z (phase += 2 x z); P, P^-1
+    // add x to z (phase += x, 3x); SUM, SUM^-1 add x[c] to x[t] and z[t] to z[c]
+    const bool h = k <= OP_HI, p = k == OP_P || k == OP_PI, s = k >= OP_SUM;
+    const uint64_t pp = p ? (uint64_t)xq : 0ull;
+    vxq = h ? zq : xq;
+    vzq = h ? xq : (zq ^ (p ? xq : (s ? zt : 0u)));
+    vxt = s ? (xt ^ xq) : xt;
+    d1 ^= (h ? (uint64_t)(xq & zq) : 0ull) ^ (k == OP_PI ? (pp & ~d0) : (pp & d0));
+    d0 ^= pp;
+    wxq = h;
+    wzq = true;
+    nb = h ? 32 : (p ? 24 : 48);
+  } else {
+    switch (k) {
+      case OP_PAULI:                               // phase += 2 (b x - a z)
+        d1 ^= (uint64_t)(((op_b(o) & 1) ? xq : 0u) ^ ((op_a(o) & 1) ? zq : 0u));
+        nb = 16;
+        break;
+      case OP_CZ:
+      case OP_CZI:                                 // z[t] ^= x[c], z[c] ^= x[t]; phase += 2 x[c] x[t]
+        vzt = zt ^ xq;
+        vzq = zq ^ xt; wzq = true;
+        d1 ^= (uint64_t)(xq & xt);
+        nb = 48;
+        break;
+      case OP_SWAP:
+        vxq = xt; vzq = zt; vxt = xq; vzt = zq; wxq = wzq = true;
+        nb = 64;
+        break;
+      case OP_NOISE: {
+        uint8_t* snap = S.noise + op_noise_slot(o) * 2 * T.rs_full;
+        ((uint64_t*)snap)[T.wofs + w] = xq;
+        ((uint64_t*)(snap + T.rs_full))[T.wofs + w] = zq;
+        nb = 16;
+        break;
+      }
     }
   }
   if (q == t) {
