@@ -1010,7 +1010,7 @@ int plan_locked(sdim_ctx* c, const sdim_job* jobs, int32_t n_jobs, int32_t mode,
         const uint32_t bpw = d == 2 ? 8 : 4;
         const uint32_t cb = gang_words(rs / bpw, K, bpw) * bpw;
         const uint32_t lds = 163584;      // not capped by lds_bytes (the one-workgroup classes' cut-over)
-        uint32_t Sg =(uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
+        uint32_t Sg = (uint32_t)std::min<int64_t>(j.shots, c->shots_per_item ? c->shots_per_item : 4096);
         auto gneed = [&](uint32_t s) -> uint64_t {
           return (uint64_t)scratch_bytes(n, cb, s, (uint32_t)gnt) + (has_noise ? noise_bytes(1, cb, s) : 0);
         };
